@@ -1,0 +1,144 @@
+/*
+ * tdhf.h -- C-ABI of the MI355X (gfx950) TDHF library, libtdhf.so.
+ *
+ * Drop-in boundary for mATTOHF's GS_calculation / RT_calculation call path.
+ * The reference has no FFI: its path is plain Fortran subroutines over
+ * module-global state (src/Global_variables.f90:5-57).  Every entry point
+ * below names the reference routine it replaces.  A Fortran caller binds them
+ * with iso_c_binding (mattohf_amd/fortran/tdhf_capi.f90).
+ *
+ * Conventions
+ *  - Orbital arrays are HOST arrays in the reference's Fortran layout
+ *    ubk(NG,NB,NK) complex(8): element (ig,ib,ik) (0-based) at offset
+ *    ig + NG*(ib + NB*ik), stored as interleaved (re,im) doubles; a
+ *    complex(c_double_complex) Fortran array can be passed directly.
+ *  - The caller owns all host buffers; the context owns device memory,
+ *    streams and graphs.  Calls are synchronous at return.
+ *  - Return value 0 = success, negative = error (TDHF_E*); the message is
+ *    available from tdhf_last_error().  No exception crosses the ABI.
+ *  - A context is used by one host thread at a time; several contexts may
+ *    coexist (each owns its own device buffers and stream).
+ *  - Everything is FP64 / complex128.
+ */
+#ifndef TDHF_H
+#define TDHF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDHF_OK 0
+#define TDHF_EINVAL (-1)  /* invalid argument or unsupported size */
+#define TDHF_ENOMEM (-2)  /* device allocation failed */
+#define TDHF_EDEVICE (-3) /* HIP runtime error or no gfx950 device */
+#define TDHF_ENUMERIC (-4) /* non-finite values detected */
+
+/* System description; mirrors src/Global_variables.f90:29-44 (NG, NK, NB,
+ * NBocc, acell) and the RT time step dt (:13).  Occupations are 2/NK for the
+ * first NBocc bands and 0 otherwise (src/Initialization.f90:176-186).
+ * Supported NG: 16, 32, 64, 128. */
+typedef struct tdhf_params {
+  int ng;
+  int nk;
+  int nb;
+  int nbocc;
+  double acell;
+  double dt;
+} tdhf_params;
+
+typedef struct tdhf_ctx tdhf_ctx;
+
+/* Creates a context for one system.  Replaces the table set-up of
+ * Initialization (src/Initialization.f90:86-146,176-186).
+ *   vion  : NG doubles, the ion potential (required; see tdhf_ion_potential)
+ *   vcg   : NG doubles vCG(G) with vCG(G=0)=0, or NULL to generate on device
+ *   vcgkk : NG*NK*NK doubles in the reference layout vCGkk(ig, ikk) with
+ *           ikk = ikr*NK + ikc, or NULL to generate the k_vCk table on the
+ *           device (src/Functions.f90:33-52).  A supplied table must depend
+ *           on ikr-ikc only; the library keeps the NG*(2NK-1) distinct rows. */
+int tdhf_create(const tdhf_params* p, const double* vion, const double* vcg,
+                const double* vcgkk, tdhf_ctx** out);
+
+/* Releases a context and all of its device memory. */
+int tdhf_destroy(tdhf_ctx* ctx);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * tdhf_create / context-free call when ctx is NULL).  Never NULL. */
+const char* tdhf_last_error(const tdhf_ctx* ctx);
+
+/* Ion potential on the device.  Replaces Prep_periodic_ion_potential
+ * (src/Prep_periodic_ion_potential.f90:10-20): images |n| <= 4*NK of
+ * -Z/sqrt(d^2+alpha^2), then shifted so that max(vion) = 0.
+ *   z, alpha, rion : nion values each (per ion; rion in units of acell). */
+int tdhf_ion_potential(const tdhf_params* p, int nion, const double* z, const double* alpha,
+                       const double* rion, double* vion);
+
+/* Coulomb tables as used by a context, for inspection: vcg (NG) and the
+ * distinct exchange-kernel rows vq(ig, d) for d = ikr-ikc+NK-1 in [0,2NK-1),
+ * i.e. NG*(2NK-1) doubles, column d at offset NG*d.  Either may be NULL.
+ * Replaces the tables of src/Initialization.f90:116-146. */
+int tdhf_get_tables(tdhf_ctx* ctx, double* vcg, double* vq);
+
+/* H[u] w.  Replaces ubkwbk_hwbk(ubk,wbk,A) (src/Functions.f90:187-209):
+ * kinetic (src/Functions.f90:212-234) + (vH[rho[u]] + vion) w + Fock
+ * exchange of w with the occupied bands of u (src/Functions.f90:237-261).
+ *   u, w, hw : NG*NB*NK complex each (hw may alias neither u nor w). */
+int tdhf_hpsi(tdhf_ctx* ctx, const double* u, const double* w, double A, double* hw);
+
+/* Observables of u at vector potential A.  Replaces ubk_Ecomps / ubk_Etot
+ * (src/Functions.f90:74-91,265-286), ubk_Je (:289-318) and the electron
+ * number of src/RT_calculation.f90:43-45.
+ *   ecomps : (Ekin, Eion, EH, Ex), may be NULL; other outputs may be NULL. */
+int tdhf_observables(tdhf_ctx* ctx, const double* u, double A, double* etot, double* ecomps,
+                     double* nele, double* je);
+
+/* One RK4 step in place.  Replaces RT_RK4_ubk(ubk,Aave,A)
+ * (src/RT_calculation.f90:154-173): stages at A(it-1)=2*Aave-A, Aave, Aave, A. */
+int tdhf_rk4_step(tdhf_ctx* ctx, double* u, double Aave, double A);
+
+/* nt RK4 steps with the observables after every step.  Replaces the loop of
+ * RT_calculation (src/RT_calculation.f90:29-60).  u is propagated in place.
+ *   A, Aave          : nt values (Define_Afield output, src/RT_calculation.f90
+ *                      :175-244); step it (0-based) uses A[it-1] (0 for it=0),
+ *                      Aave[it], A[it]
+ *   etot, nele, jtot : nt values each, observables of the state after step it
+ *                      evaluated at A[it]; may be NULL
+ *   init3            : (Etot, N, J) of the input state at A=0, may be NULL. */
+int tdhf_rt_run(tdhf_ctx* ctx, double* u, int nt, const double* A, const double* Aave,
+                double* etot, double* nele, double* jtot, double* init3);
+
+/* Device-resident variant used for timing: tdhf_state_upload copies u to the
+ * device once; tdhf_rt_propagate then runs nt steps on the resident state
+ * (observables as in tdhf_rt_run, written to host arrays at the end);
+ * tdhf_state_download copies the state back. */
+int tdhf_state_upload(tdhf_ctx* ctx, const double* u);
+int tdhf_state_download(tdhf_ctx* ctx, double* u);
+int tdhf_rt_propagate(tdhf_ctx* ctx, int nt, const double* A, const double* Aave, double* etot,
+                      double* nele, double* jtot, double* init3);
+
+/* Ground state.  Replaces GS_calculation (src/GS_calculation.f90:26-57) with
+ * KS_ubk2_SVD (:124-194), Subspace_diag (:197-237) and Gram_Schmidt
+ * (src/mATTOHF.f90:134-157).  Runs nscf iterations (or stops early once
+ * |E(iter)-E(iter-1)| < tol when tol > 0).
+ *   etot_hist : nscf+1 values (entry 0 = input orbitals), may be NULL
+ *   epsbk, epserr : NB*NK band energies / residuals of the final orbitals
+ *                   (ubk_epsbk, ubk_epserrorbk, src/Functions.f90:321-358),
+ *                   may be NULL
+ *   niter     : iterations performed, may be NULL. */
+int tdhf_gs_run(tdhf_ctx* ctx, double* u, int nscf, int ncg, double tol, double* etot_hist,
+                double* epsbk, double* epserr, int* niter);
+
+/* Gram_Schmidt (src/mATTOHF.f90:134-157) on the device, in place. */
+int tdhf_gram_schmidt(tdhf_ctx* ctx, double* u);
+
+/* Per-kernel timing for measurement: when enabled, the dominant kernel (the
+ * fused H-apply/RK4-stage kernel) is bracketed by HIP events on the stream it
+ * is launched on.  tdhf_kernel_stats returns the number of timed launches and
+ * their summed duration in milliseconds since the last reset. */
+int tdhf_set_kernel_timing(tdhf_ctx* ctx, int enable);
+int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TDHF_H */

@@ -1,0 +1,256 @@
+// tdhf_fft.h -- register-resident FP64 FFT building blocks for gfx950.
+//
+// A length-NG transform is owned by a group of L consecutive lanes of a
+// wavefront (NG = R*L, R points per lane, R >= L, R % L == 0).  Lane l holds
+// the spatial points x = L*m + l (m = 0..R-1) in registers v[m].  The
+// forward transform is an R-point DFT in registers, a twiddle by
+// exp(-2 pi i l g0 / NG), one transpose through the group's LDS tile, and
+// R/L L-point DFTs in registers; the result stays in "G layout": register
+// j = s*L + g1 of lane l holds G index g = (l*R/L + s) + R*g1.  The inverse
+// transform consumes the G layout and returns the spatial layout, so a
+// forward/inverse pair (a convolution) needs no digit reversal at all.
+//
+// Conventions are FFTW's, as used by the reference (src/Initialization.f90
+// :270-273): forward = sum_x f(x) exp(-2 pi i x g / NG), backward with +i,
+// both unnormalised.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+namespace tdhf {
+
+using zd = double2;
+
+__device__ __forceinline__ zd mk(double x, double y) {
+  zd r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+__device__ __forceinline__ zd zadd(zd a, zd b) { return mk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ zd zsub(zd a, zd b) { return mk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ zd zmul(zd a, zd b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ zd zmulc(zd a, zd b) {
+  return mk(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ zd zscal(zd a, double s) { return mk(a.x * s, a.y * s); }
+__device__ __forceinline__ double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
+
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// cos and sin of 2*pi*j/16
+template <int J>
+__device__ __forceinline__ constexpr double cos16() {
+  constexpr double t[16] = {1.0,
+                            0.92387953251128675613,
+                            0.70710678118654752440,
+                            0.38268343236508977173,
+                            0.0,
+                            -0.38268343236508977173,
+                            -0.70710678118654752440,
+                            -0.92387953251128675613,
+                            -1.0,
+                            -0.92387953251128675613,
+                            -0.70710678118654752440,
+                            -0.38268343236508977173,
+                            0.0,
+                            0.38268343236508977173,
+                            0.70710678118654752440,
+                            0.92387953251128675613};
+  return t[J & 15];
+}
+template <int J>
+__device__ __forceinline__ constexpr double sin16() {
+  return cos16<(J + 12) & 15>();  // sin(t) = cos(t - pi/2)
+}
+
+// a * exp(S * 2 pi i K / N) for compile-time K, N (N divides 16); the
+// multiplications by 1, -1 and +-i are done without arithmetic.
+template <int N, int S, int K>
+__device__ __forceinline__ zd twm(zd a) {
+  constexpr int j = (((K % N) + N) % N) * (16 / N);
+  if constexpr (j == 0) {
+    return a;
+  } else if constexpr (j == 8) {
+    return mk(-a.x, -a.y);
+  } else if constexpr (j == 4) {
+    if constexpr (S > 0)
+      return mk(-a.y, a.x);
+    else
+      return mk(a.y, -a.x);
+  } else if constexpr (j == 12) {
+    if constexpr (S > 0)
+      return mk(a.y, -a.x);
+    else
+      return mk(-a.y, a.x);
+  } else {
+    constexpr double c = cos16<j>();
+    constexpr double s = S * sin16<j>();
+    return mk(a.x * c - a.y * s, a.x * s + a.y * c);
+  }
+}
+
+// out[k] = sum_{j<N} in[IO + IS*j] * exp(S*2*pi*i*j*k/N), radix-2 DIT
+// recursion, fully unrolled; natural order in and out.
+template <int N, int S, int IS, int IO, int M>
+__device__ __forceinline__ void dft(const zd (&in)[M], zd (&out)[N]) {
+  if constexpr (N == 1) {
+    out[0] = in[IO];
+  } else {
+    zd e[N / 2], o[N / 2];
+    dft<N / 2, S, IS * 2, IO, M>(in, e);
+    dft<N / 2, S, IS * 2, IO + IS, M>(in, o);
+    static_for<0, N / 2>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const zd t = twm<N, S, k>(o[k]);
+      out[k] = zadd(e[k], t);
+      out[k + N / 2] = zsub(e[k], t);
+    });
+  }
+}
+
+// Orders the memory operations of one wavefront around an LDS exchange
+// between its lanes (lanes of a wavefront execute LDS instructions in order).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NG>
+struct Geo;
+template <>
+struct Geo<16> {
+  static constexpr int R = 4, L = 4;
+};
+template <>
+struct Geo<32> {
+  static constexpr int R = 8, L = 4;
+};
+template <>
+struct Geo<64> {
+  static constexpr int R = 8, L = 8;
+};
+template <>
+struct Geo<128> {
+  static constexpr int R = 16, L = 8;
+};
+
+template <int NG>
+struct GroupFFT {
+  static constexpr int R = Geo<NG>::R;
+  static constexpr int L = Geo<NG>::L;
+  static constexpr int Q = R / L;
+  static constexpr int PAD = L + 1;      // padded LDS row (bank spread)
+  static constexpr int TSZ = R * PAD;    // complex elements of one group's tile
+  static_assert(R * L == NG && R % L == 0, "bad FFT geometry");
+
+  zd fw[R];                    // exp(-2 pi i l g0 / NG), g0 = 0..R-1
+  zd iw[(Q == 1) ? 1 : R];     // exp(+2 pi i x0 (l*Q+s) / NG) at s*L + x0 (Q > 1)
+
+  __device__ __forceinline__ void init(int l) {
+    static_for<0, R>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      const int e = (l * g0) % NG;
+      double sn, cs;
+      sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
+      fw[g0] = mk(cs, -sn);
+    });
+    if constexpr (Q > 1) {
+      static_for<0, R>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int s = i / L, x0 = i % L;
+        const int e = (x0 * (l * Q + s)) % NG;
+        double sn, cs;
+        sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
+        iw[i] = mk(cs, sn);
+      });
+    }
+  }
+
+  // spatial layout -> G layout (forward, exp(-i...)), T = this group's tile
+  __device__ __forceinline__ void fwd(zd (&v)[R], zd* T, int l) const {
+    zd y[R];
+    dft<R, -1, 1, 0, R>(v, y);
+    static_for<1, R>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      y[g0] = zmul(y[g0], fw[g0]);
+    });
+    static_for<0, R>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      T[g0 * PAD + l] = y[g0];
+    });
+    wave_sync();
+    zd z[R];
+    static_for<0, R>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int s = i / L, x0 = i % L;
+      z[i] = T[(l * Q + s) * PAD + x0];
+    });
+    wave_sync();
+    static_for<0, Q>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[L];
+      dft<L, -1, 1, s * L, R>(z, t);
+      static_for<0, L>([&](auto gc) {
+        constexpr int g1 = decltype(gc)::value;
+        v[s * L + g1] = t[g1];
+      });
+    });
+  }
+
+  // G layout -> spatial layout (backward, exp(+i...))
+  __device__ __forceinline__ void inv(zd (&v)[R], zd* T, int l) const {
+    zd z[R];
+    static_for<0, Q>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[L];
+      dft<L, 1, 1, s * L, R>(v, t);
+      static_for<0, L>([&](auto xc) {
+        constexpr int x0 = decltype(xc)::value;
+        z[s * L + x0] = t[x0];
+      });
+    });
+    if constexpr (Q == 1) {
+      static_for<1, R>([&](auto xc) {
+        constexpr int x0 = decltype(xc)::value;
+        z[x0] = zmulc(z[x0], fw[x0]);
+      });
+    } else {
+      static_for<0, R>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        z[i] = zmul(z[i], iw[i]);
+      });
+    }
+    static_for<0, R>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      constexpr int s = i / L, x0 = i % L;
+      T[(l * Q + s) * PAD + x0] = z[i];
+    });
+    wave_sync();
+    zd y[R];
+    static_for<0, R>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      y[g0] = T[g0 * PAD + l];
+    });
+    wave_sync();
+    dft<R, 1, 1, 0, R>(y, v);
+  }
+
+  // G index held by lane l in register j of the G layout
+  __host__ __device__ static constexpr int gindex(int l, int j) {
+    return (l * Q + j / L) + R * (j % L);
+  }
+};
+
+}  // namespace tdhf

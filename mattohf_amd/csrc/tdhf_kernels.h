@@ -1,0 +1,433 @@
+// tdhf_kernels.h -- device kernels of the TDHF path (gfx950, FP64).
+//
+// The dominant kernel is apply_kernel<NG>: one workgroup per output orbital
+// (k,i) ("row").  Its groups of L lanes stream the occupied orbitals u(q,j)
+// of every k-point q and, for each, do the exchange pair round trip of
+// ubkwbk_vFwbk (src/Functions.f90:247-257):
+//     rho = w(k,i) * conj(u(q,j))  ->  FFT  ->  * -v(G+k-q)/(NG*NK)  ->  IFFT
+//     acc += result * u(q,j)
+// entirely in registers plus one LDS transpose per transform.  The same pass
+// accumulates the density sum_{q,j} |u(q,j)|^2 (ubk_dns, src/Functions.f90
+// :55-71), so no separate density kernel is needed.  After a workgroup-wide
+// reduction, wave 0 does the Hartree transform (dns_vH, :147-162) and the
+// kinetic transform of w (ubk_tubk, :212-234) side by side, and the epilogue
+// forms H w = T w + (vH + vion) w + V_F w (ubkwbk_hwbk, :187-209) and applies
+// the mode-specific update (plain H|w>, one RK4 stage of RT_RK4_ubk,
+// src/RT_calculation.f90:154-173, or observables only).  Per-row partial sums
+// of Ekin, J and Ex (src/Functions.f90:94-119,165-183,289-318) are produced
+// from the same transforms.
+#pragma once
+#include "tdhf_fft.h"
+
+namespace tdhf {
+
+enum ApplyMode : int {
+  M_HPSI = 0,  // out = H[u] w
+  M_S1 = 1,    // RK4 stage 1 (+ observables of the stage input)
+  M_S2 = 2,
+  M_S3 = 3,
+  M_S4 = 4,
+  M_OBS = 5    // observables only
+};
+
+// Offsets inside one observables slot: rows*3 partials then (N, Eion, EH).
+struct ApplyArgs {
+  int nk, nb, nbocc, nbw;  // nbw = columns of w / out per k-point
+  double bx, dv, occ, dt;
+  double A, Aobs;           // used when Aarr == nullptr
+  const double* Aarr;       // per-step A(it), or nullptr
+  const double* Aavearr;    // per-step Aave(it)
+  const int* step_base;     // device step counter (graph replay), or nullptr
+  int jstep;                // step = *step_base + jstep
+  int obs_slot;             // slot when step_base == nullptr
+  int obs_stride;           // doubles per observables slot
+  const double* kx;         // NK
+  const double* vion;       // NG
+  const double* vcgl;       // NG, G layout, vCG(G)/NG
+  const double* vtl;        // (2NK-1)*NG, G layout, -v(G+k-q)/(NG*NK)
+  const zd* u;              // (NG,NB,NK) orbitals defining rho and V_F
+  const zd* w;              // (NG,nbw,NK) orbitals H acts on
+  zd* out;                  // M_HPSI output (NG,nbw,NK)
+  zd* u0;                   // RK4: state at the start of the step
+  zd* ksum;                 // RK4: k1 + 2 k2 + 2 k3
+  zd* unext;                // RK4: next stage input
+  double* obs;              // observables slots, or nullptr
+  int mode;
+};
+
+__device__ __forceinline__ double gvalue(int g, int NG, double bx) {
+  // Gx in FFT order, src/Initialization.f90:97-102
+  return g < NG / 2 ? (double)g * bx : (double)(g - NG) * bx;
+}
+
+template <int NG>
+__global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
+  using F = GroupFFT<NG>;
+  constexpr int R = F::R, L = F::L, GPW = 64 / L;
+  extern __shared__ zd smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  const int l = lane % L, gw = lane / L;
+  const int grp = wave * GPW + gw, ngrp = nw * GPW;
+  const int row = blockIdx.x;
+  const int ik = row / a.nbw, ib = row - ik * a.nbw;
+
+  zd* T = smem + grp * F::TSZ;
+  zd* sVF = smem + ngrp * F::TSZ;
+  zd* sTW = sVF + NG;
+  double* sRho = reinterpret_cast<double*>(sTW + NG);
+  double* sVH = sRho + NG;
+  double* sRed = sVH + NG;
+
+  // field values for this launch
+  int step = 0, slot = a.obs_slot;
+  double Akin = a.A, Aobs = a.Aobs;
+  if (a.step_base) {
+    step = *a.step_base + a.jstep;
+    const double At = a.Aarr[step], Aav = a.Aavearr[step];
+    if (a.mode == M_S1) {
+      Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
+      Aobs = step > 0 ? a.Aarr[step - 1] : 0.0;
+      slot = step;
+    } else if (a.mode == M_S2 || a.mode == M_S3) {
+      Akin = Aav;
+    } else if (a.mode == M_S4) {
+      Akin = At;
+    } else {  // M_OBS after step 'step'
+      Akin = At;
+      Aobs = At;
+      slot = step + 1;
+    }
+  }
+
+  F fft;
+  fft.init(l);
+  const zd* wrow = a.w + (size_t)NG * row;
+  zd wr[R];
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    wr[m] = wrow[L * m + l];
+  });
+
+  zd acc[R];
+  double den[R];
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    acc[m] = mk(0.0, 0.0);
+    den[m] = 0.0;
+  });
+
+  // ---- exchange pair loop (ubkwbk_vFwbk, src/Functions.f90:247-257) ----
+  const int nitem = a.nk * a.nbocc;
+  for (int item = grp; item < nitem; item += ngrp) {
+    const int iq = item / a.nbocc, jb = item - iq * a.nbocc;
+    const zd* urow = a.u + (size_t)NG * (jb + a.nb * iq);
+    zd uq[R], v[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      uq[m] = urow[L * m + l];
+    });
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[m] = zmulc(wr[m], uq[m]);
+      den[m] += zabs2(uq[m]);
+    });
+    fft.fwd(v, T, l);
+    const double* vt = a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + l * R;
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      v[j] = zscal(v[j], vt[j]);
+    });
+    fft.inv(v, T, l);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      acc[m].x = fma(v[m].x, uq[m].x, acc[m].x);
+      acc[m].x = fma(-v[m].y, uq[m].y, acc[m].x);
+      acc[m].y = fma(v[m].x, uq[m].y, acc[m].y);
+      acc[m].y = fma(v[m].y, uq[m].x, acc[m].y);
+    });
+  }
+
+  // ---- reduce over the groups of the wave (xor butterflies), then waves ----
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+#pragma unroll
+    for (int off = L; off < 64; off <<= 1) {
+      acc[m].x += __shfl_xor(acc[m].x, off);
+      acc[m].y += __shfl_xor(acc[m].y, off);
+      den[m] += __shfl_xor(den[m], off);
+    }
+  });
+  {
+    // each wave parks its partial in its own (now idle) tile area
+    zd* wacc = smem + wave * GPW * F::TSZ;
+    double* wden = reinterpret_cast<double*>(wacc + NG);
+    if (gw == 0) {
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        wacc[L * m + l] = acc[m];
+        wden[L * m + l] = den[m];
+      });
+    }
+  }
+  __syncthreads();
+  if (tid < NG) {
+    zd s = mk(0.0, 0.0);
+    double d = 0.0;
+    for (int wv = 0; wv < nw; ++wv) {
+      const zd* wacc = smem + wv * GPW * F::TSZ;
+      const double* wden = reinterpret_cast<const double*>(wacc + NG);
+      s = zadd(s, wacc[tid]);
+      d += wden[tid];
+    }
+    sVF[tid] = s;
+    sRho[tid] = a.occ * d;
+  }
+  __syncthreads();
+
+  // ---- wave 0: Hartree (group 0) and kinetic (group 1) round trips ----
+  const double kxv = a.kx[ik];
+  if (wave == 0) {
+    zd v[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if (gw == 0)
+        v[m] = mk(sRho[L * m + l], 0.0);
+      else if (gw == 1)
+        v[m] = wr[m];
+      else
+        v[m] = mk(0.0, 0.0);
+    });
+    fft.fwd(v, T, l);
+    double pEk = 0.0, pJ = 0.0;
+    const double kAx = kxv + Akin, kA2 = kAx * kAx;
+    const double kAo = kxv + Aobs, kAo2 = kAo * kAo;
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const int g = F::gindex(l, j);
+      const double G = gvalue(g, NG, a.bx);
+      if (gw == 0) {
+        v[j] = zscal(v[j], a.vcgl[l * R + j]);
+      } else {
+        const double p2 = zabs2(v[j]);
+        pEk += 0.5 * (G * G + 2.0 * G * kAo + kAo2) * p2;
+        pJ += (G + kxv + Aobs) * p2;
+        const double mult = 0.5 * (G * G + 2.0 * G * kAx + kA2);
+        v[j] = zscal(v[j], mult / (double)NG);
+      }
+    });
+    fft.inv(v, T, l);
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) {
+      pEk += __shfl_xor(pEk, off);
+      pJ += __shfl_xor(pJ, off);
+    }
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if (gw == 0) sVH[L * m + l] = v[m].x;
+      if (gw == 1) sTW[L * m + l] = v[m];
+    });
+    if (gw == 1 && l == 0) {
+      sRed[0] = pEk;
+      sRed[1] = pJ;
+    }
+  }
+  __syncthreads();
+
+  // ---- epilogue: H w and the mode-specific update ----
+  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && a.obs != nullptr;
+  double pEx = 0.0;
+  if (tid < NG) {
+    const size_t o = (size_t)NG * row + tid;
+    const zd wv = a.w[o];
+    const zd vf = sVF[tid];
+    zd h = sTW[tid];
+    h = zadd(h, zscal(wv, sVH[tid] + a.vion[tid]));
+    h = zadd(h, vf);
+    pEx = wv.x * vf.x + wv.y * vf.y;
+    if (a.mode == M_HPSI) {
+      a.out[o] = h;
+    } else if (a.mode >= M_S1 && a.mode <= M_S4) {
+      const zd kk = mk(h.y, -h.x);  // -i * H w
+      const double hdt = 0.5 * a.dt;
+      if (a.mode == M_S1) {
+        a.ksum[o] = kk;
+        const zd u0 = a.u0[o];
+        a.unext[o] = mk(u0.x + hdt * kk.x, u0.y + hdt * kk.y);
+      } else if (a.mode == M_S2 || a.mode == M_S3) {
+        const zd ks = a.ksum[o];
+        a.ksum[o] = mk(ks.x + 2.0 * kk.x, ks.y + 2.0 * kk.y);
+        const zd u0 = a.u0[o];
+        const double c = (a.mode == M_S2) ? hdt : a.dt;
+        a.unext[o] = mk(u0.x + c * kk.x, u0.y + c * kk.y);
+      } else {
+        const zd ks = a.ksum[o];
+        const zd u0 = a.u0[o];
+        a.u0[o] = mk(u0.x + (ks.x + kk.x) * a.dt / 6.0, u0.y + (ks.y + kk.y) * a.dt / 6.0);
+      }
+    }
+  }
+  if (want_obs) {
+    // reduce pEx over the NG epilogue threads
+    if (tid < 64) {
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) pEx += __shfl_xor(pEx, off);
+    } else if (tid < 128) {
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) pEx += __shfl_xor(pEx, off);
+      if (tid == 64) sRed[2] = pEx;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      if (NG > 64) pEx += sRed[2];
+      const double occr = ib < a.nbocc ? a.occ : 0.0;
+      double* p = a.obs + (size_t)slot * a.obs_stride;
+      p[3 * row + 0] = occr * sRed[0] / (double)NG * a.dv;
+      p[3 * row + 1] = occr * sRed[1] / (double)NG;
+      p[3 * row + 2] = occr * pEx * a.dv;
+      if (row == 0) {
+        const int rows = a.nbw * a.nk;
+        double n = 0.0, ei = 0.0, eh = 0.0;
+        for (int x = 0; x < NG; ++x) {
+          n += sRho[x];
+          ei += a.vion[x] * sRho[x];
+          eh += sVH[x] * sRho[x];
+        }
+        p[3 * rows + 0] = n * a.dv;
+        p[3 * rows + 1] = ei * a.dv;
+        p[3 * rows + 2] = 0.5 * eh * a.dv;
+      }
+    }
+  }
+}
+
+// Sums the per-row partials of each observables slot into
+// (Etot, N, J, Ekin, Eion, EH, Ex), src/Functions.f90:74-91 and :289-318.
+__global__ void obs_reduce_kernel(const double* obs, int stride, int rows, int nslots, double dv,
+                                  double vcell, double* out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const double* p = obs + (size_t)s * stride;
+  double ek = 0.0, jj = 0.0, ex = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    ek += p[3 * r + 0];
+    jj += p[3 * r + 1];
+    ex += p[3 * r + 2];
+  }
+  ex = 0.5 * ex;
+  const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
+  double* o = out + (size_t)7 * s;
+  o[0] = ek + ei + eh + ex;
+  o[1] = n;
+  o[2] = -(jj * dv / vcell);
+  o[3] = ek;
+  o[4] = ei;
+  o[5] = eh;
+  o[6] = ex;
+}
+
+__global__ void step_advance_kernel(int* step_base, int by) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step_base += by;
+}
+
+// k_vCk (src/Functions.f90:33-52) for a list of arguments: one workgroup per
+// argument, the nsteps trapezoid terms split over the threads and summed by
+// a fixed-shape tree (deterministic).
+__global__ void __launch_bounds__(256) kvck_kernel(const double* q, double dx, long nsteps,
+                                                   double* out) {
+  __shared__ double red[256];
+  const double k = q[blockIdx.x];
+  double s = 0.0;
+  for (long ix = threadIdx.x; ix < nsteps; ix += blockDim.x) {
+    const double x0 = (double)ix * dx, x1 = (double)(ix + 1) * dx;
+    s += 0.5 * (cos(k * x0) / sqrt(x0 * x0 + 1.0) + cos(k * x1) / sqrt(x1 * x1 + 1.0));
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = 2.0 * red[0] * dx;
+}
+
+// Rearranges a raw table t[d*NG + g] into the G layout of GroupFFT<NG>,
+// scaled: out[d*NG + l*R + j] = scale * t[d*NG + gindex(l,j)].
+template <int NG>
+__global__ void relayout_kernel(const double* t, int nrows, double scale, double* out) {
+  using F = GroupFFT<NG>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows * NG) return;
+  const int d = i / NG, r = i - d * NG;
+  const int l = r / F::R, j = r - l * F::R;
+  out[i] = scale * t[(size_t)d * NG + F::gindex(l, j)];
+}
+
+// Prep_periodic_ion_potential (src/Prep_periodic_ion_potential.f90:10-20);
+// one workgroup of NG threads.
+__global__ void ion_kernel(int ng, int nk, double ax, int nion, const double* z,
+                           const double* alpha, const double* rion, double* vion) {
+  __shared__ double red[128];
+  const int x = threadIdx.x;
+  double v = 0.0;
+  if (x < ng) {
+    const double rx = (double)x / (double)ng * ax;
+    for (int ii = 0; ii < nion; ++ii)
+      for (int icell = -4 * nk; icell <= 4 * nk; ++icell) {
+        const double dist = fabs(rx - ax * rion[ii] - ax * (double)icell);
+        v = v - z[ii] / sqrt(dist * dist + alpha[ii] * alpha[ii]);
+      }
+    red[x] = v;
+  }
+  __syncthreads();
+  if (x == 0) {
+    double top = red[0];
+    for (int i = 1; i < ng; ++i) top = red[i] > top ? red[i] : top;
+    red[127] = top;
+  }
+  __syncthreads();
+  const double top = red[127];
+  __syncthreads();
+  if (x < ng) vion[x] = v - top;
+}
+
+// Block-wide deterministic sum over blockDim.x (power of two, <= 128).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+    if (t < h) red[t] += red[t + h];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Gram_Schmidt (src/mATTOHF.f90:134-157): one workgroup (NG threads) per
+// k-point, modified Gram-Schmidt with the vcell/NG weight.
+__global__ void gram_schmidt_kernel(zd* u, int ng, int nb, double vcell) {
+  __shared__ double red[2][128];
+  const int ik = blockIdx.x, x = threadIdx.x;
+  zd* uk = u + (size_t)ng * nb * ik;
+  for (int ib = 0; ib < nb; ++ib) {
+    zd ui = uk[(size_t)ng * ib + x];
+    for (int ibt = 0; ibt < ib; ++ibt) {
+      const zd ut = uk[(size_t)ng * ibt + x];
+      double re = block_sum(ut.x * ui.x + ut.y * ui.y, red[0]);
+      double im = block_sum(ut.x * ui.y - ut.y * ui.x, red[1]);
+      re = re * vcell / (double)ng;
+      im = im * vcell / (double)ng;
+      ui = mk(ui.x - (ut.x * re - ut.y * im), ui.y - (ut.x * im + ut.y * re));
+    }
+    double s = block_sum(ui.x * ui.x + ui.y * ui.y, red[0]);
+    s = s * vcell / (double)ng;
+    const double r = sqrt(s);
+    ui = mk(ui.x / r, ui.y / r);
+    uk[(size_t)ng * ib + x] = ui;
+    __syncthreads();
+  }
+}
+
+}  // namespace tdhf

@@ -1,0 +1,318 @@
+"""Python host interface of libtdhf.so (ctypes over the C-ABI of include/tdhf.h).
+
+Method names mirror the reference routines they stand in for, so that a
+caller of the reference's Fortran path finds the same operations:
+
+    ubkwbk_hwbk   src/Functions.f90:187-209      H[u] w
+    ubk_Ecomps    src/Functions.f90:265-286      (Ekin, Eion, EH, Ex)
+    ubk_Etot      src/Functions.f90:74-91
+    ubk_Je        src/Functions.f90:289-318
+    RT_RK4_ubk    src/RT_calculation.f90:154-173
+    RT_calculation src/RT_calculation.f90:2-69 (the time loop)
+    GS_calculation src/GS_calculation.f90:2-121
+    Gram_Schmidt  src/mATTOHF.f90:134-157
+
+Arrays are complex128 in the reference layout ubk(NG,NB,NK) (Fortran order).
+The compute path is the HIP library only: if libtdhf.so is missing or the
+device is not a gfx950 GPU, the calls raise TDHFError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libtdhf.so")
+_lib = None
+
+
+class TDHFError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [("ng", C.c_int), ("nk", C.c_int), ("nb", C.c_int), ("nbocc", C.c_int),
+                ("acell", C.c_double), ("dt", C.c_double)]
+
+
+# Every entry point of include/tdhf.h with its ctypes signature.
+_vp, _d, _i = C.c_void_p, C.c_double, C.c_int
+_PP = C.POINTER(Params)
+SIGNATURES = {
+    "tdhf_create": (_i, [_PP, _vp, _vp, _vp, C.POINTER(_vp)]),
+    "tdhf_destroy": (_i, [_vp]),
+    "tdhf_last_error": (C.c_char_p, [_vp]),
+    "tdhf_ion_potential": (_i, [_PP, _i, _vp, _vp, _vp, _vp]),
+    "tdhf_get_tables": (_i, [_vp, _vp, _vp]),
+    "tdhf_hpsi": (_i, [_vp, _vp, _vp, _d, _vp]),
+    "tdhf_observables": (_i, [_vp, _vp, _d, _vp, _vp, _vp, _vp]),
+    "tdhf_rk4_step": (_i, [_vp, _vp, _d, _d]),
+    "tdhf_rt_run": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tdhf_state_upload": (_i, [_vp, _vp]),
+    "tdhf_state_download": (_i, [_vp, _vp]),
+    "tdhf_rt_propagate": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tdhf_gs_run": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, C.POINTER(_i)]),
+    "tdhf_gram_schmidt": (_i, [_vp, _vp]),
+    "tdhf_set_kernel_timing": (_i, [_vp, _i]),
+    "tdhf_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
+}
+
+
+def load(path: str | None = None):
+    """Loads libtdhf.so and declares the signatures of include/tdhf.h."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise TDHFError(f"{p} is missing: build it with `python -m mattohf_amd.build`")
+    L = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(rc, h=None):
+    if rc != 0:
+        msg = load().tdhf_last_error(h)
+        raise TDHFError(f"libtdhf error {rc}: {msg.decode() if msg else ''}")
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+# ---------------------------------------------------------------------------
+# set-up helpers (host side of the driver, as in the reference's Program Main)
+# ---------------------------------------------------------------------------
+def grids(ng, nk, acell):
+    """rx, Gx, kx of src/Initialization.f90:93-112."""
+    bx = 2.0 * math.pi / acell
+    rx = np.array([i / ng * acell for i in range(ng)])
+    gx = np.array([(i if i < ng // 2 else i - ng) * bx for i in range(ng)])
+    kx = np.array([(-0.5 + (2.0 * (i + 1) - 1.0) / (2 * nk)) * bx for i in range(nk)])
+    return rx, gx, kx
+
+
+def init_wf(ng, nb, nk, acell):
+    """Init_wf + quickrnd (src/mATTOHF.f90:159-196): LCG seed 123, k outer."""
+    u = np.zeros((ng, nb, nk), dtype=np.complex128, order="F")
+    rx = np.array([i / ng * acell for i in range(ng)])
+    iseed = 123
+    for ik in range(nk):
+        for ib in range(nb):
+            iseed = (iseed * 106 + 1283) % 6075
+            rxc = (iseed / 6075.0) * acell
+            u[:, ib, ik] = [math.exp(-0.5 * ((x - rxc) * (x - rxc))) for x in rx]
+    return u
+
+
+def ion_potential(ng, nk, acell, z, alpha, rion):
+    """Prep_periodic_ion_potential on the device (tdhf_ion_potential)."""
+    z, alpha, rion = (np.ascontiguousarray(np.atleast_1d(x), dtype=np.float64)
+                      for x in (z, alpha, rion))
+    p = Params(ng, nk, 1, 1, float(acell), 0.0)
+    v = np.empty(ng)
+    _chk(load().tdhf_ion_potential(C.byref(p), len(z), _ptr(z), _ptr(alpha), _ptr(rion),
+                                   _ptr(v)))
+    return v
+
+
+HARTREE = 27.2116
+ATOMTIME = 0.02419
+ATOMFIELD = 514.2
+
+
+def define_afield_lp(nt, dt, omegac_ev, tpulse_fs, e0_vnm, phicep, nenvelope):
+    """Define_Afield, field_option 'LP' (src/RT_calculation.f90:186-234)."""
+    time = dt * np.arange(nt, dtype=np.float64)
+    A = np.zeros(nt)
+    omegac = omegac_ev / HARTREE
+    tpulse = tpulse_fs / ATOMTIME
+    e0 = e0_vnm / ATOMFIELD
+    aamp = e0 / omegac
+    ntp = int(min(tpulse / dt, float(nt)))
+    for it in range(ntp):
+        env = math.sin(math.pi * time[it] / tpulse)
+        p = 1.0
+        for _ in range(nenvelope):
+            p *= env
+        A[it] = aamp * math.sin(omegac * (time[it] - 0.5 * tpulse) + 2.0 * math.pi * phicep) * p
+    Aave = np.empty(nt)
+    E = np.empty(nt)
+    prev = 0.0
+    for it in range(nt):
+        Aave[it] = 0.5 * (A[it] + prev)
+        E[it] = -(A[it] - prev) / dt
+        prev = A[it]
+    return time, A, Aave, E
+
+
+def define_afield_lr(nt, dt, dAc):
+    """Define_Afield, field_option 'LR': impulsive kick A = dAc."""
+    time = dt * np.arange(nt, dtype=np.float64)
+    A = np.full(nt, float(dAc))
+    Aave = np.empty(nt)
+    E = np.empty(nt)
+    prev = 0.0
+    for it in range(nt):
+        Aave[it] = 0.5 * (A[it] + prev)
+        E[it] = -(A[it] - prev) / dt
+        prev = A[it]
+    return time, A, Aave, E
+
+
+def pulse_lp_params(A0, omega, T):
+    """Maps A(t) = A0 sin^2(pi t/T) sin(omega t) onto Define_Afield's 'LP'
+    inputs: (omegac [eV], Tpulse [fs], E0 [V/nm], phiCEP, nenvelope=2)."""
+    return omega * HARTREE, T * ATOMTIME, A0 * omega * ATOMFIELD, omega * T / (4 * math.pi), 2
+
+
+# ---------------------------------------------------------------------------
+# the context
+# ---------------------------------------------------------------------------
+class TDHF:
+    """One system on the GPU (a tdhf_ctx)."""
+
+    def __init__(self, ng, nk, nb, nbocc, acell, dt, vion, vcg=None, vcgkk=None):
+        self.ng, self.nk, self.nb, self.nbocc = int(ng), int(nk), int(nb), int(nbocc)
+        self.acell, self.dt = float(acell), float(dt)
+        self.params = Params(self.ng, self.nk, self.nb, self.nbocc, self.acell, self.dt)
+        vion = _f64(vion, self.ng)
+        vcg = None if vcg is None else _f64(vcg, self.ng)
+        vcgkk = None if vcgkk is None else _f64(vcgkk, self.ng * self.nk * self.nk)
+        h = C.c_void_p()
+        _chk(load().tdhf_create(C.byref(self.params), _ptr(vion), _ptr(vcg), _ptr(vcgkk),
+                                C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().tdhf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.ng, self.nb, self.nk)
+
+    def _u(self, u, copy=False):
+        u = np.asarray(u, dtype=np.complex128)
+        if u.shape != self.shape:
+            u = u.reshape(self.shape, order="F")
+        u = np.asfortranarray(u)
+        return u.copy(order="F") if copy else u
+
+    def tables(self):
+        vcg = np.empty(self.ng)
+        vq = np.empty(self.ng * (2 * self.nk - 1))
+        _chk(load().tdhf_get_tables(self.h, _ptr(vcg), _ptr(vq)), self.h)
+        return vcg, vq.reshape(2 * self.nk - 1, self.ng)
+
+    def ubkwbk_hwbk(self, ubk, wbk, A):
+        u, w = self._u(ubk), self._u(wbk)
+        out = np.empty(self.shape, dtype=np.complex128, order="F")
+        _chk(load().tdhf_hpsi(self.h, _ptr(u), _ptr(w), float(A), _ptr(out)), self.h)
+        return out
+
+    def observables(self, ubk, A):
+        u = self._u(ubk)
+        e, n, j = C.c_double(), C.c_double(), C.c_double()
+        ec = np.empty(4)
+        _chk(load().tdhf_observables(self.h, _ptr(u), float(A), C.byref(e), _ptr(ec),
+                                     C.byref(n), C.byref(j)), self.h)
+        return e.value, ec, n.value, j.value
+
+    def ubk_Ecomps(self, ubk, A):
+        return self.observables(ubk, A)[1]
+
+    def ubk_Etot(self, ubk, A):
+        return self.observables(ubk, A)[0]
+
+    def ubk_Je(self, ubk, A):
+        return self.observables(ubk, A)[3]
+
+    def nele(self, ubk):
+        return self.observables(ubk, 0.0)[2]
+
+    def RT_RK4_ubk(self, ubk, Aave, A):
+        u = self._u(ubk, copy=True)
+        _chk(load().tdhf_rk4_step(self.h, _ptr(u), float(Aave), float(A)), self.h)
+        return u
+
+    def RT_calculation(self, ubk, A, Aave):
+        """nt RK4 steps; returns (ubk, Etot, Nele, Jtot, (E0, N0, J0))."""
+        u = self._u(ubk, copy=True)
+        A, Aave = _f64(A), _f64(Aave, np.size(A))
+        nt = A.size
+        e, n, j, i3 = np.empty(nt), np.empty(nt), np.empty(nt), np.empty(3)
+        _chk(load().tdhf_rt_run(self.h, _ptr(u), nt, _ptr(A), _ptr(Aave), _ptr(e), _ptr(n),
+                                _ptr(j), _ptr(i3)), self.h)
+        return u, e, n, j, i3
+
+    def upload(self, ubk):
+        u = self._u(ubk)
+        _chk(load().tdhf_state_upload(self.h, _ptr(u)), self.h)
+
+    def download(self):
+        u = np.empty(self.shape, dtype=np.complex128, order="F")
+        _chk(load().tdhf_state_download(self.h, _ptr(u)), self.h)
+        return u
+
+    def propagate(self, A, Aave, want_obs=True):
+        A, Aave = _f64(A), _f64(Aave, np.size(A))
+        nt = A.size
+        e, n, j, i3 = (np.empty(nt), np.empty(nt), np.empty(nt), np.empty(3)) if want_obs \
+            else (None, None, None, None)
+        _chk(load().tdhf_rt_propagate(self.h, nt, _ptr(A), _ptr(Aave), _ptr(e), _ptr(n),
+                                      _ptr(j), _ptr(i3)), self.h)
+        return e, n, j, i3
+
+    def GS_calculation(self, ubk, nscf, ncg, tol=0.0):
+        """Returns (ubk, Etot history[0..niter], epsbk(NB,NK), epserrorbk(NB,NK))."""
+        u = self._u(ubk, copy=True)
+        eh = np.empty(nscf + 1)
+        eps = np.empty((self.nb, self.nk), order="F")
+        err = np.empty((self.nb, self.nk), order="F")
+        nit = C.c_int()
+        _chk(load().tdhf_gs_run(self.h, _ptr(u), int(nscf), int(ncg), float(tol), _ptr(eh),
+                                _ptr(eps), _ptr(err), C.byref(nit)), self.h)
+        return u, eh[: nit.value + 1], eps, err
+
+    def Gram_Schmidt(self, ubk):
+        u = self._u(ubk, copy=True)
+        _chk(load().tdhf_gram_schmidt(self.h, _ptr(u)), self.h)
+        return u
+
+    def set_kernel_timing(self, on):
+        _chk(load().tdhf_set_kernel_timing(self.h, 1 if on else 0), self.h)
+
+    def kernel_stats(self):
+        n, ms = C.c_long(), C.c_double()
+        _chk(load().tdhf_kernel_stats(self.h, C.byref(n), C.byref(ms)), self.h)
+        return n.value, ms.value

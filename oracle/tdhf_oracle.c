@@ -1,0 +1,791 @@
+/*
+ * tdhf_oracle.c -- plain-C restatement of mATTOHF's real-time TDHF path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP implementation is
+ * compared against: it is loaded by tests/, by __graft_entry__.smoke() and by
+ * bench.py's CPU-baseline leg, and by nothing else.  The product library
+ * (mattohf_amd/libtdhf.so) neither links nor calls it.
+ *
+ * Every function cites the reference routine it restates (paths relative to
+ * the reference repository).  Loop orders and the order of floating-point
+ * operations follow the reference where that is free; where the reference
+ * calls a third-party library the restatement substitutes its own routine:
+ *   - FFTW3 dfftw_execute_dft (src/Functions.f90:111,113,157,159,228,230,251,
+ *     253,308,310): own radix-2 FFT, same sign and normalisation convention;
+ *   - LAPACK ZGESVD('O') (src/GS_calculation.f90:162): modified Gram-Schmidt,
+ *     applied twice, in the dvcell-weighted inner product (same span);
+ *   - LAPACK ZHEEV (src/GS_calculation.f90:183,226): cyclic complex Jacobi,
+ *     eigenvalues ascending.
+ *
+ * How each part is pinned (what the tests check it against) is listed in the
+ * header comment of tests/test_oracle.py.
+ */
+#include "tdhf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double complex zc;
+
+/* src/Constants.f90:5-18 */
+static const double PI_ = 3.14159265358979323846264338327950288;
+static const double HARTREE_EV = 27.2116;
+static const double ATOMTIME_FS = 0.02419;
+static const double ATOMFIELD_VNM = 514.2;
+
+struct orc_ctx {
+  orc_sys s;
+  double dvcell, vcell, bx;
+  double *rx, *gx, *kx, *vion, *vcg, *vcgkk, *occ;
+};
+
+/* ------------------------------------------------------------------------ */
+/* FFT (FFTW convention: out[g] = sum_x in[x] exp(sign*2*pi*i*x*g/n))       */
+/* ------------------------------------------------------------------------ */
+static zc* tw_tab[20];
+
+static const zc* get_tw(int L) {
+  if (!tw_tab[L]) {
+#pragma omp critical(orc_tw)
+    {
+      if (!tw_tab[L]) {
+        int n = 1 << L;
+        zc* t = (zc*)malloc(sizeof(zc) * (size_t)(n > 1 ? n / 2 : 1));
+        for (int k = 0; k < n / 2; ++k) {
+          double ang = -2.0 * PI_ * (double)k / (double)n;
+          t[k] = cos(ang) + I * sin(ang);
+        }
+        __sync_synchronize();
+        tw_tab[L] = t;
+      }
+    }
+  }
+  return tw_tab[L];
+}
+
+static void fft_z(int n, int sign, const zc* in, zc* out) {
+  int L = 0;
+  while ((1 << L) < n) ++L;
+  if ((1 << L) != n) { /* brute-force DFT for other lengths */
+    for (int g = 0; g < n; ++g) {
+      zc acc = 0;
+      for (int x = 0; x < n; ++x) {
+        long ph = ((long)x * g) % n;
+        double ang = sign * 2.0 * PI_ * (double)ph / (double)n;
+        acc += in[x] * (cos(ang) + I * sin(ang));
+      }
+      out[g] = acc;
+    }
+    return;
+  }
+  const zc* tw = get_tw(L);
+  for (int i = 0; i < n; ++i) {
+    int r = 0;
+    for (int b = 0; b < L; ++b) r |= ((i >> b) & 1) << (L - 1 - b);
+    out[r] = in[i];
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    int half = len / 2, step = n / len;
+    for (int i = 0; i < n; i += len)
+      for (int j = 0; j < half; ++j) {
+        zc w = tw[j * step];
+        if (sign > 0) w = conj(w);
+        zc a = out[i + j], b = out[i + j + half] * w;
+        out[i + j] = a + b;
+        out[i + j + half] = a - b;
+      }
+  }
+}
+
+void orc_fft(int n, int sign, const double* in, double* out) {
+  fft_z(n, sign, (const zc*)in, (zc*)out);
+}
+
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grids: src/Initialization.f90:46-55 (cell), :93-112 (rx, Gx, kx)         */
+/* ------------------------------------------------------------------------ */
+void orc_grids(const orc_sys* s, double* rx, double* gx, double* kx) {
+  const int NG = s->ng, NK = s->nk;
+  const double bx = 2.0 * PI_ / s->acell;
+  for (int i = 0; i < NG; ++i) rx[i] = (double)i / (double)NG * s->acell;
+  for (int i = 0; i < NG / 2; ++i) gx[i] = (double)i * bx;
+  for (int i = NG / 2; i < NG; ++i) gx[i] = (double)(i - NG) * bx;
+  for (int i = 0; i < NK; ++i) kx[i] = (-0.5 + (2.0 * (i + 1) - 1.0) / (double)(2 * NK)) * bx;
+}
+
+/* k_vCk, src/Functions.f90:33-52: trapezoid rule for
+ *   2 * int_0^{NK*acell} cos(k x)/sqrt(x^2+1) dx,  dx = acell/NG/20,
+ * with the interval count fixed by the floating-point test ix*dx < NK*acell. */
+long orc_kvck_nsteps(const orc_sys* s) {
+  const double dx = s->acell / (double)s->ng / 20.0;
+  const double L = (double)s->nk * s->acell;
+  long ix = 0;
+  while ((double)ix * dx < L) ++ix;
+  return ix;
+}
+
+double orc_k_vck(const orc_sys* s, double k) {
+  const double dx = s->acell / (double)s->ng / 20.0;
+  const double L = (double)s->nk * s->acell;
+  double v = 0.0;
+  long ix = 0;
+  while ((double)ix * dx < L) {
+    double x0 = (double)ix * dx, x1 = (double)(ix + 1) * dx;
+    v = v + 0.5 * (cos(k * x0) / sqrt(x0 * x0 + 1.0) + cos(k * x1) / sqrt(x1 * x1 + 1.0));
+    ++ix;
+  }
+  return 2.0 * v * dx;
+}
+
+/* vCG table, src/Initialization.f90:143-146 (G=0 entry zeroed) */
+void orc_vcg(const orc_sys* s, double* vcg) {
+  double *rx = malloc(sizeof(double) * s->ng), *gx = malloc(sizeof(double) * s->ng),
+         *kx = malloc(sizeof(double) * s->nk);
+  orc_grids(s, rx, gx, kx);
+#pragma omp parallel for schedule(dynamic)
+  for (int ig = 0; ig < s->ng; ++ig) vcg[ig] = orc_k_vck(s, gx[ig]);
+  vcg[0] = 0.0;
+  free(rx); free(gx); free(kx);
+}
+
+/* vCGkk table, src/Initialization.f90:116-142: vCGkk(ig,ikk) =
+ * k_vCk(Gx(ig)+kx(ikr)-kx(ikc)), ikk = ikr*NK + ikc (0-based; row k outer).
+ * The value depends on ikr-ikc only, up to the rounding of the argument, so
+ * each of the 2NK-1 distinct differences is evaluated once (the tests check
+ * this against the per-pair argument at small NK). */
+void orc_vcgkk_full(const orc_sys* s, double* vcgkk) {
+  const int NG = s->ng, NK = s->nk;
+  double *rx = malloc(sizeof(double) * NG), *gx = malloc(sizeof(double) * NG),
+         *kx = malloc(sizeof(double) * NK);
+  orc_grids(s, rx, gx, kx);
+  /* evaluate each distinct difference once; the argument is formed from the
+   * first (ikr,ikc) pair with that difference, as the reference forms it */
+  double* uniq = malloc(sizeof(double) * (size_t)NG * (2 * NK - 1));
+#pragma omp parallel for collapse(2) schedule(dynamic)
+  for (int d = -(NK - 1); d <= NK - 1; ++d)
+    for (int ig = 0; ig < NG; ++ig) {
+      int ikr = d >= 0 ? d : 0, ikc = d >= 0 ? 0 : -d;
+      uniq[(size_t)(d + NK - 1) * NG + ig] = orc_k_vck(s, gx[ig] + kx[ikr] - kx[ikc]);
+    }
+  for (int ikr = 0; ikr < NK; ++ikr)
+    for (int ikc = 0; ikc < NK; ++ikc)
+      memcpy(vcgkk + (size_t)NG * (ikr * NK + ikc), uniq + (size_t)(ikr - ikc + NK - 1) * NG,
+             sizeof(double) * NG);
+  free(uniq); free(rx); free(gx); free(kx);
+}
+
+/* Prep_periodic_ion_potential, src/Prep_periodic_ion_potential.f90:10-20 */
+void orc_ion_potential(const orc_sys* s, int nion, const double* z, const double* alpha,
+                       const double* rion, double* vion) {
+  const int NG = s->ng, NK = s->nk;
+  const double ax = s->acell;
+  for (int ig = 0; ig < NG; ++ig) vion[ig] = 0.0;
+  for (int ii = 0; ii < nion; ++ii)
+    for (int icell = -4 * NK; icell <= 4 * NK; ++icell)
+      for (int ig = 0; ig < NG; ++ig) {
+        double rx = (double)ig / (double)NG * ax;
+        double dist = fabs(rx - ax * rion[ii] - ax * (double)icell);
+        vion[ig] = vion[ig] - z[ii] / sqrt(dist * dist + alpha[ii] * alpha[ii]);
+      }
+  double top = vion[0];
+  for (int ig = 1; ig < NG; ++ig) if (vion[ig] > top) top = vion[ig];
+  for (int ig = 0; ig < NG; ++ig) vion[ig] = vion[ig] - top;
+}
+
+/* Init_wf + quickrnd, src/mATTOHF.f90:159-196 (LCG seed 123, k outer) */
+void orc_init_wf(const orc_sys* s, double* ubk_) {
+  zc* ubk = (zc*)ubk_;
+  const int NG = s->ng, NB = s->nb, NK = s->nk;
+  int iseed = 123;
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      iseed = (iseed * 106 + 1283) % 6075;
+      double rnd = (double)iseed / 6075.0;
+      double rxc = rnd * s->acell;
+      for (int ig = 0; ig < NG; ++ig) {
+        double rx = (double)ig / (double)NG * s->acell;
+        double r2 = (rx - rxc) * (rx - rxc);
+        ubk[ig + NG * (ib + NB * ik)] = exp(-0.5 * r2);
+      }
+    }
+}
+
+/* Gram_Schmidt, src/mATTOHF.f90:134-157 (modified GS, vcell/NG weight) */
+void orc_gram_schmidt(const orc_sys* s, double* ubk_) {
+  zc* ubk = (zc*)ubk_;
+  const int NG = s->ng, NB = s->nb, NK = s->nk;
+  const double vcell = fabs(s->acell);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      zc* u = ubk + NG * (ib + NB * ik);
+      for (int ibt = 0; ibt < ib; ++ibt) {
+        const zc* t = ubk + NG * (ibt + NB * ik);
+        zc zov = 0;
+        for (int ig = 0; ig < NG; ++ig) zov += conj(t[ig]) * u[ig];
+        zov = zov * vcell / (double)NG;
+        for (int ig = 0; ig < NG; ++ig) u[ig] = u[ig] - t[ig] * zov;
+      }
+      double sn = 0;
+      for (int ig = 0; ig < NG; ++ig) sn += creal(u[ig]) * creal(u[ig]) + cimag(u[ig]) * cimag(u[ig]);
+      sn = sn * vcell / (double)NG;
+      double r = sqrt(sn);
+      for (int ig = 0; ig < NG; ++ig) u[ig] = u[ig] / r;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Context (module-global state of src/Global_variables.f90:29-57)          */
+/* ------------------------------------------------------------------------ */
+orc_ctx* orc_create(const orc_sys* s, const double* vion, const double* vcgkk_or_null) {
+  orc_ctx* c = (orc_ctx*)calloc(1, sizeof(orc_ctx));
+  c->s = *s;
+  const int NG = s->ng, NK = s->nk, NB = s->nb;
+  c->vcell = fabs(s->acell);
+  c->dvcell = c->vcell / (double)NG;
+  c->bx = 2.0 * PI_ / s->acell;
+  c->rx = malloc(sizeof(double) * NG);
+  c->gx = malloc(sizeof(double) * NG);
+  c->kx = malloc(sizeof(double) * NK);
+  orc_grids(s, c->rx, c->gx, c->kx);
+  c->vion = malloc(sizeof(double) * NG);
+  memcpy(c->vion, vion, sizeof(double) * NG);
+  c->vcg = malloc(sizeof(double) * NG);
+  orc_vcg(s, c->vcg);
+  c->vcgkk = malloc(sizeof(double) * (size_t)NG * NK * NK);
+  if (vcgkk_or_null)
+    memcpy(c->vcgkk, vcgkk_or_null, sizeof(double) * (size_t)NG * NK * NK);
+  else
+    orc_vcgkk_full(s, c->vcgkk);
+  /* occupations, src/Initialization.f90:176-186 */
+  c->occ = malloc(sizeof(double) * NB * NK);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) c->occ[ib + NB * ik] = ib < s->nbocc ? 2.0 / (double)NK : 0.0;
+  return c;
+}
+
+void orc_destroy(orc_ctx* c) {
+  if (!c) return;
+  free(c->rx); free(c->gx); free(c->kx); free(c->vion); free(c->vcg); free(c->vcgkk); free(c->occ);
+  free(c);
+}
+
+/* ubk_dns, src/Functions.f90:55-71 (ig outer, ik, ib inner) */
+void orc_dns(const orc_ctx* c, const double* ubk_, double* dns) {
+  const zc* ubk = (const zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  for (int ig = 0; ig < NG; ++ig) {
+    double d = 0.0;
+    for (int ik = 0; ik < NK; ++ik)
+      for (int ib = 0; ib < NB; ++ib) {
+        zc u = ubk[ig + NG * (ib + NB * ik)];
+        d = d + c->occ[ib + NB * ik] * creal(conj(u) * u);
+      }
+    dns[ig] = d;
+  }
+}
+
+/* dns_vH, src/Functions.f90:147-162 */
+void orc_vh(const orc_ctx* c, const double* dns, double* vh) {
+  const int NG = c->s.ng;
+  zc* w1 = malloc(sizeof(zc) * NG);
+  zc* w2 = malloc(sizeof(zc) * NG);
+  for (int i = 0; i < NG; ++i) w1[i] = dns[i];
+  fft_z(NG, -1, w1, w2);
+  for (int i = 0; i < NG; ++i) w2[i] = c->vcg[i] * w2[i];
+  fft_z(NG, +1, w2, w1);
+  for (int i = 0; i < NG; ++i) vh[i] = creal(w1[i]) / (double)NG;
+  free(w1); free(w2);
+}
+
+/* ubk_tubk, src/Functions.f90:212-234 */
+void orc_tpsi(const orc_ctx* c, const double* wbk_, double A, double* out_) {
+  const zc* wbk = (const zc*)wbk_;
+  zc* out = (zc*)out_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+#pragma omp parallel
+  {
+    zc* work = malloc(sizeof(zc) * NG);
+#pragma omp for
+    for (int ik = 0; ik < NK; ++ik) {
+      double kAx = c->kx[ik] + A, kA2 = kAx * kAx;
+      for (int ib = 0; ib < NB; ++ib) {
+        fft_z(NG, -1, wbk + NG * (ib + NB * ik), work);
+        for (int ig = 0; ig < NG; ++ig) {
+          double G = c->gx[ig];
+          work[ig] = 0.5 * (G * G + 2.0 * G * kAx + kA2) * work[ig] / (double)NG;
+        }
+        fft_z(NG, +1, work, out + NG * (ib + NB * ik));
+      }
+    }
+    free(work);
+  }
+}
+
+/* ubkwbk_vFwbk, src/Functions.f90:237-261: Fock exchange of w with the
+ * occupied orbitals of u.  One "rank" owns all NK^2 k-pairs here; rows are
+ * parallelised by output k, preserving the reference's summation order. */
+void orc_vfpsi(const orc_ctx* c, const double* ubk_, const double* wbk_, double* out_) {
+  const zc* ubk = (const zc*)ubk_;
+  const zc* wbk = (const zc*)wbk_;
+  zc* out = (zc*)out_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk, NBocc = c->s.nbocc;
+  memset(out, 0, sizeof(zc) * (size_t)NG * NB * NK);
+#pragma omp parallel
+  {
+    zc* pr = malloc(sizeof(zc) * NG);
+    zc* pg = malloc(sizeof(zc) * NG);
+    zc* vf = malloc(sizeof(zc) * NG);
+#pragma omp for schedule(dynamic)
+    for (int ikr = 0; ikr < NK; ++ikr)
+      for (int ikc = 0; ikc < NK; ++ikc) {
+        const double* vk = c->vcgkk + (size_t)NG * (ikr * NK + ikc);
+        for (int ibr = 0; ibr < NB; ++ibr)
+          for (int ibc = 0; ibc < NBocc; ++ibc) {
+            const zc* w = wbk + NG * (ibr + NB * ikr);
+            const zc* u = ubk + NG * (ibc + NB * ikc);
+            for (int ig = 0; ig < NG; ++ig) pr[ig] = w[ig] * conj(u[ig]) / (double)NK;
+            fft_z(NG, -1, pr, pg);
+            for (int ig = 0; ig < NG; ++ig) pg[ig] = -vk[ig] * pg[ig] / (double)NG;
+            fft_z(NG, +1, pg, vf);
+            zc* o = out + NG * (ibr + NB * ikr);
+            for (int ig = 0; ig < NG; ++ig) o[ig] = o[ig] + vf[ig] * u[ig];
+          }
+      }
+    free(pr); free(pg); free(vf);
+  }
+}
+
+/* ubkwbk_hwbk, src/Functions.f90:187-209 */
+void orc_hpsi(const orc_ctx* c, const double* ubk, const double* wbk_, double A, double* out_) {
+  const zc* wbk = (const zc*)wbk_;
+  zc* out = (zc*)out_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  const size_t n = (size_t)NG * NB * NK;
+  orc_tpsi(c, wbk_, A, out_);
+  double* dns = malloc(sizeof(double) * NG);
+  double* vh = malloc(sizeof(double) * NG);
+  orc_dns(c, ubk, dns);
+  orc_vh(c, dns, vh);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib)
+      for (int ig = 0; ig < NG; ++ig) {
+        size_t o = ig + NG * (ib + NB * ik);
+        out[o] = out[o] + (vh[ig] + c->vion[ig]) * wbk[o];
+      }
+  zc* vf = malloc(sizeof(zc) * n);
+  orc_vfpsi(c, ubk, wbk_, (double*)vf);
+  for (size_t i = 0; i < n; ++i) out[i] = out[i] + vf[i];
+  free(vf); free(dns); free(vh);
+}
+
+/* ubk_Ekin, src/Functions.f90:94-119 */
+static double ekin(const orc_ctx* c, const zc* ubk, double A) {
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  zc* work = malloc(sizeof(zc) * NG);
+  zc* work2 = malloc(sizeof(zc) * NG);
+  double e = 0.0;
+  for (int ik = 0; ik < NK; ++ik) {
+    double kAx = c->kx[ik] + A, kA2 = kAx * kAx;
+    for (int ib = 0; ib < NB; ++ib) {
+      const zc* u = ubk + NG * (ib + NB * ik);
+      fft_z(NG, -1, u, work);
+      for (int ig = 0; ig < NG; ++ig) {
+        double G = c->gx[ig];
+        work[ig] = 0.5 * (G * G + 2.0 * G * kAx + kA2) * work[ig] / (double)NG;
+      }
+      fft_z(NG, +1, work, work2);
+      zc sacc = 0;
+      for (int ig = 0; ig < NG; ++ig) sacc += conj(u[ig]) * work2[ig];
+      e = e + creal(c->occ[ib + NB * ik] * sacc * c->dvcell);
+    }
+  }
+  free(work); free(work2);
+  return e;
+}
+
+/* ubk_Ecomps / ubk_Etot, src/Functions.f90:74-91,265-286 with dns_Eion
+ * :122-131, dns_EH :134-144 and ubk_Ex :165-183 */
+void orc_ecomps(const orc_ctx* c, const double* ubk_, double A, double* ec) {
+  const zc* ubk = (const zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk, NBocc = c->s.nbocc;
+  double* dns = malloc(sizeof(double) * NG);
+  double* vh = malloc(sizeof(double) * NG);
+  orc_dns(c, ubk_, dns);
+  double Ek = ekin(c, ubk, A);
+  double Eion = 0.0;
+  for (int ig = 0; ig < NG; ++ig) Eion += c->vion[ig] * dns[ig];
+  Eion = Eion * c->dvcell;
+  orc_vh(c, dns, vh);
+  double EH = 0.0;
+  for (int ig = 0; ig < NG; ++ig) EH += vh[ig] * dns[ig];
+  EH = 0.5 * EH * c->dvcell;
+  zc* vf = malloc(sizeof(zc) * (size_t)NG * NB * NK);
+  orc_vfpsi(c, ubk_, ubk_, (double*)vf);
+  double Ex = 0.0;
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NBocc; ++ib) {
+      zc sacc = 0;
+      for (int ig = 0; ig < NG; ++ig)
+        sacc += conj(ubk[ig + NG * (ib + NB * ik)]) * vf[ig + NG * (ib + NB * ik)];
+      Ex = Ex + creal(c->occ[ib + NB * ik] * sacc * c->dvcell);
+    }
+  Ex = 0.5 * Ex;
+  ec[0] = Ek; ec[1] = Eion; ec[2] = EH; ec[3] = Ex;
+  free(vf); free(dns); free(vh);
+}
+
+double orc_etot(const orc_ctx* c, const double* ubk, double A) {
+  double ec[4];
+  orc_ecomps(c, ubk, A, ec);
+  return ec[0] + ec[1] + ec[2] + ec[3];
+}
+
+/* ubk_Je, src/Functions.f90:289-318 */
+double orc_je(const orc_ctx* c, const double* ubk_, double A) {
+  const zc* ubk = (const zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  zc* w1 = malloc(sizeof(zc) * NG);
+  zc* w2 = malloc(sizeof(zc) * NG);
+  zc* w3 = malloc(sizeof(zc) * NG);
+  zc J = 0;
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      const zc* u = ubk + NG * (ib + NB * ik);
+      fft_z(NG, -1, u, w1);
+      for (int ig = 0; ig < NG; ++ig) w1[ig] = w1[ig] / (double)NG;
+      for (int ig = 0; ig < NG; ++ig) w3[ig] = (c->gx[ig] + c->kx[ik] + A) * w1[ig];
+      fft_z(NG, +1, w3, w2);
+      zc sacc = 0;
+      for (int ig = 0; ig < NG; ++ig) sacc += conj(u[ig]) * w2[ig];
+      J = J + c->occ[ib + NB * ik] * sacc;
+    }
+  J = J * c->dvcell / c->vcell;
+  free(w1); free(w2); free(w3);
+  return -creal(J);
+}
+
+/* N = sum(dns)*dvcell, src/RT_calculation.f90:43-45 */
+double orc_nele(const orc_ctx* c, const double* ubk) {
+  double* dns = malloc(sizeof(double) * c->s.ng);
+  orc_dns(c, ubk, dns);
+  double n = 0.0;
+  for (int ig = 0; ig < c->s.ng; ++ig) n += dns[ig];
+  free(dns);
+  return n * c->dvcell;
+}
+
+/* ubk_epsbk / ubk_epserrorbk, src/Functions.f90:321-358 (A=0) */
+void orc_epsbk(const orc_ctx* c, const double* ubk_, double* eps, double* err) {
+  const zc* ubk = (const zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  zc* hu = malloc(sizeof(zc) * (size_t)NG * NB * NK);
+  orc_hpsi(c, ubk_, ubk_, 0.0, (double*)hu);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      const zc* u = ubk + NG * (ib + NB * ik);
+      zc* h = hu + NG * (ib + NB * ik);
+      zc sacc = 0;
+      for (int ig = 0; ig < NG; ++ig) sacc += conj(u[ig]) * h[ig];
+      double e = creal(sacc) * c->dvcell;
+      if (eps) eps[ib + NB * ik] = e;
+      double sa = 0.0;
+      for (int ig = 0; ig < NG; ++ig) sa += cabs(h[ig] - e * u[ig]);
+      if (err) err[ib + NB * ik] = sqrt(sa * c->dvcell);
+    }
+  free(hu);
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK4 and the RT loop                                                      */
+/* ------------------------------------------------------------------------ */
+/* RT_RK4_ubk, src/RT_calculation.f90:154-173 */
+void orc_rk4_step(const orc_ctx* c, double* ubk_, double dt, double Aave, double A) {
+  zc* u = (zc*)ubk_;
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  zc *k1 = malloc(sizeof(zc) * n), *k2 = malloc(sizeof(zc) * n), *k3 = malloc(sizeof(zc) * n),
+     *k4 = malloc(sizeof(zc) * n), *t = malloc(sizeof(zc) * n), *h = malloc(sizeof(zc) * n);
+  const double Am = 2.0 * Aave - A;
+  const double hdt = 0.5 * dt;
+  orc_hpsi(c, ubk_, ubk_, Am, (double*)h);
+  for (size_t i = 0; i < n; ++i) k1[i] = -I * h[i];
+  for (size_t i = 0; i < n; ++i) t[i] = u[i] + hdt * k1[i];
+  orc_hpsi(c, (double*)t, (double*)t, Aave, (double*)h);
+  for (size_t i = 0; i < n; ++i) k2[i] = -I * h[i];
+  for (size_t i = 0; i < n; ++i) t[i] = u[i] + hdt * k2[i];
+  orc_hpsi(c, (double*)t, (double*)t, Aave, (double*)h);
+  for (size_t i = 0; i < n; ++i) k3[i] = -I * h[i];
+  for (size_t i = 0; i < n; ++i) t[i] = u[i] + dt * k3[i];
+  orc_hpsi(c, (double*)t, (double*)t, A, (double*)h);
+  for (size_t i = 0; i < n; ++i) k4[i] = -I * h[i];
+  for (size_t i = 0; i < n; ++i) u[i] = u[i] + (k1[i] + 2.0 * k2[i] + 2.0 * k3[i] + k4[i]) * dt / 6.0;
+  free(k1); free(k2); free(k3); free(k4); free(t); free(h);
+}
+
+/* RT_calculation time loop, src/RT_calculation.f90:29-60.  init3 receives
+ * (Etot, N, J) of the initial state evaluated at A=0 (Etotm1, Nelem1, Jtotm1). */
+void orc_rt_run(const orc_ctx* c, double* ubk, double dt, int nt, const double* A,
+                const double* Aave, double* etot, double* nele, double* jtot, double* init3) {
+  if (init3) {
+    init3[0] = orc_etot(c, ubk, 0.0);
+    init3[1] = orc_nele(c, ubk);
+    init3[2] = orc_je(c, ubk, 0.0);
+  }
+  for (int it = 0; it < nt; ++it) {
+    orc_rk4_step(c, ubk, dt, Aave[it], A[it]);
+    if (nele) nele[it] = orc_nele(c, ubk);
+    if (etot) etot[it] = orc_etot(c, ubk, A[it]);
+    if (jtot) jtot[it] = orc_je(c, ubk, A[it]);
+  }
+}
+
+/* Define_Afield, src/RT_calculation.f90:175-244 (field_option 'LP') */
+void orc_define_afield_lp(int nt, double dt, double omegac, double tpulse, double e0,
+                          double phicep, int nenv, double* time, double* A, double* Aave,
+                          double* E) {
+  for (int it = 0; it < nt; ++it) { time[it] = dt * (double)it; A[it] = 0.0; }
+  omegac = omegac / HARTREE_EV;
+  tpulse = tpulse / ATOMTIME_FS;
+  e0 = e0 / ATOMFIELD_VNM;
+  const double Aamp = e0 / omegac;
+  double lim = tpulse / dt < (double)nt ? tpulse / dt : (double)nt;
+  int ntp = (int)lim;
+  for (int it = 0; it < ntp; ++it) {
+    double env = sin(PI_ * time[it] / tpulse), p = 1.0;
+    for (int j = 0; j < nenv; ++j) p *= env;
+    A[it] = Aamp * sin(omegac * (time[it] - 0.5 * tpulse) + 2.0 * PI_ * phicep) * p;
+  }
+  Aave[0] = 0.5 * (A[0] + 0.0);
+  for (int it = 1; it < nt; ++it) Aave[it] = 0.5 * (A[it] + A[it - 1]);
+  if (E) {
+    E[0] = -(A[0] - 0.0) / dt;
+    for (int it = 1; it < nt; ++it) E[it] = -(A[it] - A[it - 1]) / dt;
+  }
+}
+
+/* Define_Afield, field_option 'LR' (impulsive kick A = dAc) */
+void orc_define_afield_lr(int nt, double dt, double dAc, double* time, double* A, double* Aave,
+                          double* E) {
+  for (int it = 0; it < nt; ++it) { time[it] = dt * (double)it; A[it] = dAc; }
+  Aave[0] = 0.5 * (A[0] + 0.0);
+  for (int it = 1; it < nt; ++it) Aave[it] = 0.5 * (A[it] + A[it - 1]);
+  if (E) {
+    E[0] = -(A[0] - 0.0) / dt;
+    for (int it = 1; it < nt; ++it) E[it] = -(A[it] - A[it - 1]) / dt;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Ground state                                                             */
+/* ------------------------------------------------------------------------ */
+/* Hermitian eigenproblem, standing in for ZHEEV('V','U') (src/GS_calculation.f90
+ * :183,226).  a: m x m column-major complex (both triangles set), overwritten by
+ * the eigenvectors (columns); eigenvalues ascending.  Cyclic Jacobi: for each
+ * (p,q), W = diag(1,e^{-i phi}) * [[c,s],[-s,c]] with a_pq = r e^{i phi} and
+ * (c,s) the symmetric Schur rotation of [[a_pp,r],[r,a_qq]]. */
+int orc_heev(int m, double* a_, double* eigs) {
+  zc* a = (zc*)a_;
+  zc* v = (zc*)calloc((size_t)m * m, sizeof(zc));
+  for (int i = 0; i < m; ++i) v[i + (size_t)m * i] = 1.0;
+  for (int i = 0; i < m; ++i) a[i + (size_t)m * i] = creal(a[i + (size_t)m * i]);
+  int sweep, ok = 0;
+  for (sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+        double x = cabs(a[i + (size_t)m * j]);
+        tot += x * x;
+        if (i != j) off += x * x;
+      }
+    if (off <= 1e-32 * tot) { ok = 1; break; }
+    for (int p = 0; p < m - 1; ++p)
+      for (int q = p + 1; q < m; ++q) {
+        zc apq = a[p + (size_t)m * q];
+        double r = cabs(apq);
+        if (r == 0.0) continue;
+        zc ph = apq / r;
+        double app = creal(a[p + (size_t)m * p]), aqq = creal(a[q + (size_t)m * q]);
+        double tau = (aqq - app) / (2.0 * r);
+        double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+        zc eph = conj(ph); /* e^{-i phi} */
+        for (int i = 0; i < m; ++i) { /* A <- A W */
+          zc aip = a[i + (size_t)m * p], aiq = a[i + (size_t)m * q];
+          a[i + (size_t)m * p] = cs * aip - sn * eph * aiq;
+          a[i + (size_t)m * q] = sn * aip + cs * eph * aiq;
+          zc vip = v[i + (size_t)m * p], viq = v[i + (size_t)m * q];
+          v[i + (size_t)m * p] = cs * vip - sn * eph * viq;
+          v[i + (size_t)m * q] = sn * vip + cs * eph * viq;
+        }
+        for (int j = 0; j < m; ++j) { /* A <- W^H A */
+          zc apj = a[p + (size_t)m * j], aqj = a[q + (size_t)m * j];
+          a[p + (size_t)m * j] = cs * apj - sn * ph * aqj;
+          a[q + (size_t)m * j] = sn * apj + cs * ph * aqj;
+        }
+        a[p + (size_t)m * q] = 0.0;
+        a[q + (size_t)m * p] = 0.0;
+        a[p + (size_t)m * p] = creal(a[p + (size_t)m * p]);
+        a[q + (size_t)m * q] = creal(a[q + (size_t)m * q]);
+      }
+  }
+  /* sort ascending (selection sort keeps it simple for m <= 64) */
+  int* idx = (int*)malloc(sizeof(int) * m);
+  for (int i = 0; i < m; ++i) { idx[i] = i; eigs[i] = creal(a[i + (size_t)m * i]); }
+  for (int i = 0; i < m; ++i) {
+    int b = i;
+    for (int j = i + 1; j < m; ++j) if (eigs[idx[j]] < eigs[idx[b]]) b = j;
+    int t = idx[i]; idx[i] = idx[b]; idx[b] = t;
+  }
+  double* ev = (double*)malloc(sizeof(double) * m);
+  for (int i = 0; i < m; ++i) ev[i] = eigs[idx[i]];
+  for (int i = 0; i < m; ++i) eigs[i] = ev[i];
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i < m; ++i) a[i + (size_t)m * j] = v[i + (size_t)m * idx[j]];
+  free(ev); free(idx); free(v);
+  return ok ? 0 : 1;
+}
+
+/* dvcell-weighted modified Gram-Schmidt of the m columns of an NG x m block,
+ * applied twice.  Stands in for the left singular vectors of ZGESVD('O')
+ * rescaled by 1/sqrt(dvcell) (src/GS_calculation.f90:161-166): both give a
+ * dvcell-orthonormal basis of the same span, and the Rayleigh-Ritz step that
+ * follows depends on the span only. */
+static void mgs_block(int NG, int m, double dv, zc* X) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int c = 0; c < m; ++c) {
+      zc* x = X + (size_t)NG * c;
+      for (int b = 0; b < c; ++b) {
+        const zc* y = X + (size_t)NG * b;
+        zc ov = 0;
+        for (int ig = 0; ig < NG; ++ig) ov += conj(y[ig]) * x[ig];
+        ov *= dv;
+        for (int ig = 0; ig < NG; ++ig) x[ig] -= ov * y[ig];
+      }
+      double s = 0;
+      for (int ig = 0; ig < NG; ++ig) s += creal(x[ig]) * creal(x[ig]) + cimag(x[ig]) * cimag(x[ig]);
+      s = sqrt(s * dv);
+      for (int ig = 0; ig < NG; ++ig) x[ig] /= s;
+    }
+}
+
+/* KS_ubk2_SVD, src/GS_calculation.f90:124-194 (eshift = 0, A = 0) */
+void orc_ks_krylov(const orc_ctx* c, double* ubk_, int ncg) {
+  zc* ubk = (zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  const size_t nblk = (size_t)NG * NB * NK;
+  const int m = NB * ncg;
+  zc* work = (zc*)malloc(sizeof(zc) * nblk * ncg);  /* block iCG at work + iCG*nblk */
+  zc* hwork = (zc*)malloc(sizeof(zc) * nblk * ncg);
+  memcpy(work, ubk, sizeof(zc) * nblk);
+  for (int icg = 1; icg < ncg; ++icg)
+    orc_hpsi(c, ubk_, (double*)(work + (icg - 1) * nblk), 0.0, (double*)(work + icg * nblk));
+  zc* X = (zc*)malloc(sizeof(zc) * (size_t)NG * m);
+  for (int ik = 0; ik < NK; ++ik) {
+    for (int icg = 0; icg < ncg; ++icg)
+      for (int ib = 0; ib < NB; ++ib)
+        memcpy(X + (size_t)NG * (NB * icg + ib), work + icg * nblk + NG * (ib + NB * ik),
+               sizeof(zc) * NG);
+    mgs_block(NG, m, c->dvcell, X);
+    for (int icg = 0; icg < ncg; ++icg)
+      for (int ib = 0; ib < NB; ++ib)
+        memcpy(work + icg * nblk + NG * (ib + NB * ik), X + (size_t)NG * (NB * icg + ib),
+               sizeof(zc) * NG);
+  }
+  for (int icg = 0; icg < ncg; ++icg)
+    orc_hpsi(c, ubk_, (double*)(work + icg * nblk), 0.0, (double*)(hwork + icg * nblk));
+  zc* whw = (zc*)malloc(sizeof(zc) * (size_t)m * m);
+  double* eigs = (double*)malloc(sizeof(double) * m);
+  for (int ik = 0; ik < NK; ++ik) {
+#define COL(arr, cc) ((arr) + ((cc) / NB) * nblk + NG * ((cc) % NB + NB * ik))
+    for (int r = 0; r < m; ++r) {
+      for (int cc = 0; cc < r; ++cc) {
+        zc sacc = 0;
+        const zc *x = COL(work, r), *h = COL(hwork, cc);
+        for (int ig = 0; ig < NG; ++ig) sacc += conj(x[ig]) * h[ig];
+        whw[r + (size_t)m * cc] = sacc * c->dvcell;
+        whw[cc + (size_t)m * r] = conj(whw[r + (size_t)m * cc]);
+      }
+      zc sacc = 0;
+      const zc *x = COL(work, r), *h = COL(hwork, r);
+      for (int ig = 0; ig < NG; ++ig) sacc += conj(x[ig]) * h[ig];
+      whw[r + (size_t)m * r] = creal(sacc) * c->dvcell;
+    }
+    orc_heev(m, (double*)whw, eigs);
+    for (int ib = 0; ib < NB; ++ib) {
+      zc* o = ubk + NG * (ib + NB * ik);
+      for (int ig = 0; ig < NG; ++ig) o[ig] = 0;
+      for (int cc = 0; cc < m; ++cc) {
+        const zc* x = COL(work, cc);
+        zc coef = whw[cc + (size_t)m * ib];
+        for (int ig = 0; ig < NG; ++ig) o[ig] += x[ig] * coef;
+      }
+    }
+#undef COL
+  }
+  free(whw); free(eigs); free(X); free(work); free(hwork);
+}
+
+/* Subspace_diag, src/GS_calculation.f90:197-237 */
+void orc_subspace_diag(const orc_ctx* c, double* ubk_) {
+  zc* ubk = (zc*)ubk_;
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  const size_t n = (size_t)NG * NB * NK;
+  zc* hu = (zc*)malloc(sizeof(zc) * n);
+  zc* out = (zc*)calloc(n, sizeof(zc));
+  orc_hpsi(c, ubk_, ubk_, 0.0, (double*)hu);
+  zc* mat = (zc*)malloc(sizeof(zc) * NB * NB);
+  double* eigs = (double*)malloc(sizeof(double) * NB);
+  for (int ik = 0; ik < NK; ++ik) {
+    for (int r = 0; r < NB; ++r) {
+      const zc* x = ubk + NG * (r + NB * ik);
+      for (int cc = 0; cc < r; ++cc) {
+        const zc* h = hu + NG * (cc + NB * ik);
+        zc sacc = 0;
+        for (int ig = 0; ig < NG; ++ig) sacc += conj(x[ig]) * h[ig];
+        mat[r + NB * cc] = sacc * c->dvcell;
+        mat[cc + NB * r] = conj(mat[r + NB * cc]);
+      }
+      const zc* h = hu + NG * (r + NB * ik);
+      zc sacc = 0;
+      for (int ig = 0; ig < NG; ++ig) sacc += conj(x[ig]) * h[ig];
+      mat[r + NB * r] = creal(sacc) * c->dvcell;
+    }
+    orc_heev(NB, (double*)mat, eigs);
+    for (int r = 0; r < NB; ++r)
+      for (int cc = 0; cc < NB; ++cc) {
+        const zc* x = ubk + NG * (cc + NB * ik);
+        zc* o = out + NG * (r + NB * ik);
+        zc coef = mat[cc + NB * r];
+        for (int ig = 0; ig < NG; ++ig) o[ig] += x[ig] * coef;
+      }
+  }
+  memcpy(ubk, out, sizeof(zc) * n);
+  free(hu); free(out); free(mat); free(eigs);
+}
+
+/* GS_calculation, src/GS_calculation.f90:26-43: etot_hist[0] is the energy of
+ * the input orbitals; etot_hist[iter] after each of the nscf iterations.
+ * epsbk/epserr (NB x NK) are evaluated on the final orbitals. */
+void orc_gs_run(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_hist,
+                double* epsbk, double* epserr) {
+  if (etot_hist) etot_hist[0] = orc_etot(c, ubk, 0.0);
+  for (int iter = 1; iter <= nscf; ++iter) {
+    orc_gram_schmidt(&c->s, ubk);
+    orc_ks_krylov(c, ubk, ncg);
+    orc_gram_schmidt(&c->s, ubk);
+    orc_subspace_diag(c, ubk);
+    if (etot_hist) etot_hist[iter] = orc_etot(c, ubk, 0.0);
+  }
+  if (epsbk || epserr) orc_epsbk(c, ubk, epsbk, epserr);
+}

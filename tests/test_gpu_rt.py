@@ -1,0 +1,83 @@
+"""GPU parity of the run-level paths (GS_calculation, RT_calculation) on the
+CPU-runnable system (a=8, NG=32, NK=8, Z=4, 2 doubly occupied bands).
+
+The RT pulse is the sin^2 pulse A0=0.2, w=0.057, T=100 expressed through
+Define_Afield's 'LP' inputs.  The time step is 0.025 au: 0.05 au exceeds the
+RK4 stability bound of this grid (|lambda_max| dt ~ 4.5 > 2*sqrt(2); both the
+oracle and the reference algorithm diverge there), 0.025 au gives ~2.25.
+Both codes start from the same ground-state orbitals (computed by the oracle).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mattohf_amd import tdhf as T
+
+pytestmark = pytest.mark.gpu
+
+NG, NK, NB, NBOCC, ACELL, Z = 32, 8, 2, 2, 8.0, 4.0
+DT, NT = 0.025, 2000
+
+
+@pytest.fixture(scope="module")
+def system():
+    S = po.System(NG, NK, NB, NBOCC, ACELL)
+    vion = S.ion_potential([Z], [1.0], [0.0])
+    O = po.Oracle(S, vion)
+    u0 = S.gram_schmidt(S.init_wf())
+    ugs, eh, eps, err = O.GS_calculation(u0, 250, 4)
+    G = T.TDHF(NG, NK, NB, NBOCC, ACELL, DT, vion)
+    yield dict(S=S, O=O, G=G, u0=u0, ugs=ugs, eh=eh, eps=eps, vion=vion)
+    G.close()
+
+
+def test_init_wf_matches_oracle(system):
+    S = system["S"]
+    assert np.abs(T.init_wf(NG, NB, NK, ACELL) - S.init_wf()).max() == 0.0
+
+
+def test_gs_eigenvalues(system):
+    G, u0 = system["G"], system["u0"]
+    u, eh, eps, err = G.GS_calculation(u0, 250, 4)
+    assert abs(eh[-1] - system["eh"][-1]) < 1e-10
+    assert np.abs(eps - system["eps"]).max() < 1e-10
+    # the converged orbitals give the same density
+    O = system["O"]
+    assert np.abs(O.ubk_dns(u) - O.ubk_dns(system["ugs"])).max() < 1e-8
+
+
+def test_gs_energy_history_first_iterations(system):
+    G, O, u0 = system["G"], system["O"], system["u0"]
+    _, eh_g, _, _ = G.GS_calculation(u0, 5, 4)
+    _, eh_o, _, _ = O.GS_calculation(u0, 5, 4)
+    assert np.abs(eh_g - eh_o).max() < 1e-9
+
+
+def test_rt_trajectory(system):
+    G, O, ugs = system["G"], system["O"], system["ugs"]
+    lp = T.pulse_lp_params(0.2, 0.057, 100.0)
+    _, A, Aave, _ = T.define_afield_lp(NT, DT, *lp)
+    _, A_o, Aave_o, _ = po.define_afield_lp(NT, DT, *lp)
+    assert np.abs(A - A_o).max() < 1e-15
+    u_g, e_g, n_g, j_g, i3_g = G.RT_calculation(ugs, A, Aave)
+    u_o, e_o, n_o, j_o, i3_o = O.RT_calculation(ugs, DT, A, Aave)
+    assert np.abs(i3_g - i3_o).max() < 1e-10
+    assert np.abs(e_g - e_o).max() < 1e-10
+    assert np.abs(n_g - n_o).max() < 1e-10
+    assert np.abs(j_g - j_o).max() / np.abs(j_o).max() < 1e-8
+    assert np.abs(j_o).max() > 1e-3  # the pulse drives a real current
+
+
+def test_rt_chunked_equals_single(system):
+    """Graph replays (25-step chunks) and the remainder path give one trajectory."""
+    G, ugs = system["G"], system["ugs"]
+    lp = T.pulse_lp_params(0.2, 0.057, 100.0)
+    _, A, Aave, _ = T.define_afield_lp(60, DT, *lp)
+    _, e1, n1, j1, _ = G.RT_calculation(ugs, A, Aave)
+    u = ugs.copy(order="F")
+    js = []
+    for it in range(60):
+        u = G.RT_RK4_ubk(u, Aave[it], A[it])
+        js.append(G.ubk_Je(u, A[it]))
+    np.testing.assert_allclose(np.array(js), j1, rtol=1e-12, atol=1e-16)
+    np.testing.assert_allclose(u, G.RT_calculation(ugs, A, Aave)[0], rtol=0, atol=1e-14)
