@@ -63,7 +63,7 @@ def setup(cfg, gs_iters=60):
     return G, vion, u, eh
 
 
-def cpu_baseline(cfg, vion, G, u, A, Aave, budget_s=12.0, max_steps=40):
+def cpu_baseline(cfg, vion, G, u, A, Aave, budget_s=12.0, max_steps=2000):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
@@ -95,6 +95,21 @@ def cpu_baseline(cfg, vion, G, u, A, Aave, budget_s=12.0, max_steps=40):
                       f"OpenMP {threads} threads, {dt:.1f} s"}
 
 
+def copy_probe(nbytes=1 << 30, reps=4):
+    """A plain device-to-device copy of nbytes, reps times (torch's copy kernel):
+    under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE it gives the counter reading
+    of a kernel whose HBM traffic is known (nbytes read + nbytes written)."""
+    import torch
+
+    n = nbytes // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,7 +118,12 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=25)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the path has no multi-GPU mode); "
               "use --gpus 1", file=sys.stderr)

@@ -38,6 +38,7 @@ struct tdhf_ctx {
   // device tables
   double *d_kx = nullptr, *d_vion = nullptr, *d_vcg = nullptr, *d_vcgl = nullptr;
   double *d_vq = nullptr, *d_vtl = nullptr;
+  zd* d_tw = nullptr;
   // orbital buffers (NG*NB*NK complex each)
   zd *d_u0 = nullptr, *d_ksum = nullptr, *d_sa = nullptr, *d_sb = nullptr;
   zd *d_w = nullptr, *d_out = nullptr;
@@ -132,10 +133,7 @@ void dfree(T*& p) {
 // Apply-kernel dispatch over the supported grid sizes.
 template <int NG>
 size_t lds_bytes(int nw) {
-  using F = tdhf::GroupFFT<NG>;
-  const int ngrp = nw * (64 / F::L);
-  return (size_t)ngrp * F::TSZ * sizeof(zd) + 2 * NG * sizeof(zd) + 2 * NG * sizeof(double) +
-         16 * sizeof(double);
+  return tdhf::apply_lds_bytes<NG>(nw);
 }
 
 size_t lds_for(int ng, int nw) {
@@ -218,6 +216,7 @@ tdhf::ApplyArgs base_args(const tdhf_ctx* c) {
   a.vion = c->d_vion;
   a.vcgl = c->d_vcgl;
   a.vtl = c->d_vtl;
+  a.tw = c->d_tw;
   a.obs_stride = c->obs_stride;
   return a;
 }
@@ -458,6 +457,14 @@ int build_tables(tdhf_ctx* c, const double* vcg, const double* vcgkk) {
   }
   CHKRET(dalloc(c, &c->d_vcgl, NG));
   CHKRET(dalloc(c, &c->d_vtl, (size_t)NG * ND));
+  CHKRET(dalloc(c, &c->d_tw, 2 * NG));
+  switch (NG) {
+    case 16: hipLaunchKernelGGL(tdhf::twiddle_kernel<16>, dim3(1), dim3(256), 0, c->stream, c->d_tw); break;
+    case 32: hipLaunchKernelGGL(tdhf::twiddle_kernel<32>, dim3(1), dim3(256), 0, c->stream, c->d_tw); break;
+    case 64: hipLaunchKernelGGL(tdhf::twiddle_kernel<64>, dim3(1), dim3(256), 0, c->stream, c->d_tw); break;
+    case 128: hipLaunchKernelGGL(tdhf::twiddle_kernel<128>, dim3(1), dim3(256), 0, c->stream, c->d_tw); break;
+  }
+  HIPCHK(c, hipGetLastError());
   CHKRET(relayout(c, c->d_vcg, 1, 1.0 / (double)NG, c->d_vcgl));
   CHKRET(relayout(c, c->d_vq, ND, -1.0 / ((double)NG * (double)NK), c->d_vtl));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -469,7 +476,7 @@ void release(tdhf_ctx* c) {
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   c->ev_pool.clear();
   dfree(c->d_kx); dfree(c->d_vion); dfree(c->d_vcg); dfree(c->d_vcgl);
-  dfree(c->d_vq); dfree(c->d_vtl);
+  dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw);
   dfree(c->d_u0); dfree(c->d_ksum); dfree(c->d_sa); dfree(c->d_sb);
   dfree(c->d_w); dfree(c->d_out);
   dfree(c->d_obs); dfree(c->d_res); dfree(c->d_A); dfree(c->d_Aave); dfree(c->d_step);

@@ -158,24 +158,35 @@ struct GroupFFT {
   zd fw[R];                    // exp(-2 pi i l g0 / NG), g0 = 0..R-1
   zd iw[(Q == 1) ? 1 : R];     // exp(+2 pi i x0 (l*Q+s) / NG) at s*L + x0 (Q > 1)
 
-  __device__ __forceinline__ void init(int l) {
+  // Loads this lane's twiddles from the table built by twiddle_kernel:
+  // tw[l*R + g0] = fw of lane l, tw[NG + l*R + i] = iw of lane l.
+  __device__ __forceinline__ void init(const zd* tw, int l) {
     static_for<0, R>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
-      const int e = (l * g0) % NG;
-      double sn, cs;
-      sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
-      fw[g0] = mk(cs, -sn);
+      fw[g0] = tw[l * R + g0];
     });
     if constexpr (Q > 1) {
       static_for<0, R>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        constexpr int s = i / L, x0 = i % L;
-        const int e = (x0 * (l * Q + s)) % NG;
-        double sn, cs;
-        sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
-        iw[i] = mk(cs, sn);
+        iw[i] = tw[NG + l * R + i];
       });
     }
+  }
+
+  // Value of the twiddle tables (used once, by twiddle_kernel).
+  __device__ static zd table_value(int idx) {
+    double sn, cs;
+    if (idx < NG) {
+      const int l = idx / R, g0 = idx % R;
+      const int e = (l * g0) % NG;
+      sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
+      return mk(cs, -sn);
+    }
+    const int r = idx - NG, l = r / R, i = r % R;
+    const int s = i / L, x0 = i % L;
+    const int e = (x0 * (l * Q + s)) % NG;
+    sincospi(2.0 * (double)e / (double)NG, &sn, &cs);
+    return mk(cs, sn);
   }
 
   // spatial layout -> G layout (forward, exp(-i...)), T = this group's tile

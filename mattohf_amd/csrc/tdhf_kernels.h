@@ -45,6 +45,7 @@ struct ApplyArgs {
   const double* vion;       // NG
   const double* vcgl;       // NG, G layout, vCG(G)/NG
   const double* vtl;        // (2NK-1)*NG, G layout, -v(G+k-q)/(NG*NK)
+  const zd* tw;             // twiddle table of GroupFFT<NG> (2*NG)
   const zd* u;              // (NG,NB,NK) orbitals defining rho and V_F
   const zd* w;              // (NG,nbw,NK) orbitals H acts on
   zd* out;                  // M_HPSI output (NG,nbw,NK)
@@ -60,10 +61,20 @@ __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
   return g < NG / 2 ? (double)g * bx : (double)(g - NG) * bx;
 }
 
+// LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' tiles
+// (transposes during the pair loop, reduction slots after it), then w, V_F w,
+// T w (complex NG each), rho and vH (NG doubles each) and a small scratch.
+template <int NG>
+__host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
+  return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
+         3 * NG * sizeof(zd) + 2 * NG * sizeof(double) + 16 * sizeof(double);
+}
+
 template <int NG>
 __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
+  static_assert(2 * NG <= 2 * F::TSZ, "reduction slot must fit in a tile");
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -73,7 +84,8 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
   zd* T = smem + grp * F::TSZ;
-  zd* sVF = smem + ngrp * F::TSZ;
+  zd* sW = smem + ngrp * F::TSZ;
+  zd* sVF = sW + NG;
   zd* sTW = sVF + NG;
   double* sRho = reinterpret_cast<double*>(sTW + NG);
   double* sVH = sRho + NG;
@@ -100,16 +112,24 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
+  // ---- prologue: w of this row into LDS; epilogue operands into registers ----
+  const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
+  zd wv = mk(0.0, 0.0), u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0);
+  double vionv = 0.0;
+  if (tid < NG) {
+    wv = a.w[o];
+    sW[tid] = wv;
+    vionv = a.vion[tid];
+    if (a.mode >= M_S1 && a.mode <= M_S4) {
+      u0v = a.u0[o];
+      if (a.mode != M_S1) ksv = a.ksum[o];
+    }
+  }
+  const double kxv = a.kx[ik];
   F fft;
-  fft.init(l);
-  const zd* wrow = a.w + (size_t)NG * row;
-  zd wr[R];
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    wr[m] = wrow[L * m + l];
-  });
+  fft.init(a.tw, l);
 
-  zd acc[R];
+  zd acc[R], un[R];
   double den[R];
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
@@ -118,25 +138,41 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   });
 
   // ---- exchange pair loop (ubkwbk_vFwbk, src/Functions.f90:247-257) ----
+  // u(q,j) of the next pair is fetched while the current pair is transformed.
   const int nitem = a.nk * a.nbocc;
-  for (int item = grp; item < nitem; item += ngrp) {
+  auto fetch = [&](int item) {
     const int iq = item / a.nbocc, jb = item - iq * a.nbocc;
     const zd* urow = a.u + (size_t)NG * (jb + a.nb * iq);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      un[m] = urow[L * m + l];
+    });
+  };
+  if (grp < nitem) fetch(grp);
+  __syncthreads();  // sW visible
+  for (int item = grp; item < nitem; item += ngrp) {
     zd uq[R], v[R];
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      uq[m] = urow[L * m + l];
+      uq[m] = un[m];
     });
+    if (item + ngrp < nitem) fetch(item + ngrp);
+    const int iq = item / a.nbocc;
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      v[m] = zmulc(wr[m], uq[m]);
+      v[m] = zmulc(sW[L * m + l], uq[m]);
       den[m] += zabs2(uq[m]);
     });
-    fft.fwd(v, T, l);
     const double* vt = a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + l * R;
+    double vtr[R];
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      v[j] = zscal(v[j], vt[j]);
+      vtr[j] = vt[j];
+    });
+    fft.fwd(v, T, l);
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      v[j] = zscal(v[j], vtr[j]);
     });
     fft.inv(v, T, l);
     static_for<0, R>([&](auto mc) {
@@ -148,45 +184,42 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     });
   }
 
-  // ---- reduce over the groups of the wave (xor butterflies), then waves ----
+  // Hartree multipliers for the tail transform (wave 0, group 0)
+  double vcg[R];
+  static_for<0, R>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    vcg[j] = (wave == 0 && gw == 0) ? a.vcgl[l * R + j] : 0.0;
+  });
+
+  // ---- reduction over all groups through LDS (fixed order) ----
+  // round 1: V_F w; each group parks its partial in its own tile
+  double* slotd = reinterpret_cast<double*>(T);
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-#pragma unroll
-    for (int off = L; off < 64; off <<= 1) {
-      acc[m].x += __shfl_xor(acc[m].x, off);
-      acc[m].y += __shfl_xor(acc[m].y, off);
-      den[m] += __shfl_xor(den[m], off);
-    }
+    slotd[2 * (L * m + l)] = acc[m].x;
+    slotd[2 * (L * m + l) + 1] = acc[m].y;
   });
-  {
-    // each wave parks its partial in its own (now idle) tile area
-    zd* wacc = smem + wave * GPW * F::TSZ;
-    double* wden = reinterpret_cast<double*>(wacc + NG);
-    if (gw == 0) {
-      static_for<0, R>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        wacc[L * m + l] = acc[m];
-        wden[L * m + l] = den[m];
-      });
-    }
+  __syncthreads();
+  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
+    reinterpret_cast<double*>(sVF)[idx] = s;
   }
   __syncthreads();
-  if (tid < NG) {
-    zd s = mk(0.0, 0.0);
-    double d = 0.0;
-    for (int wv = 0; wv < nw; ++wv) {
-      const zd* wacc = smem + wv * GPW * F::TSZ;
-      const double* wden = reinterpret_cast<const double*>(wacc + NG);
-      s = zadd(s, wacc[tid]);
-      d += wden[tid];
-    }
-    sVF[tid] = s;
-    sRho[tid] = a.occ * d;
+  // round 2: density
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    slotd[L * m + l] = den[m];
+  });
+  __syncthreads();
+  for (int idx = tid; idx < NG; idx += blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
+    sRho[idx] = a.occ * s;
   }
   __syncthreads();
 
   // ---- wave 0: Hartree (group 0) and kinetic (group 1) round trips ----
-  const double kxv = a.kx[ik];
   if (wave == 0) {
     zd v[R];
     static_for<0, R>([&](auto mc) {
@@ -194,7 +227,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       if (gw == 0)
         v[m] = mk(sRho[L * m + l], 0.0);
       else if (gw == 1)
-        v[m] = wr[m];
+        v[m] = sW[L * m + l];
       else
         v[m] = mk(0.0, 0.0);
     });
@@ -207,7 +240,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       const int g = F::gindex(l, j);
       const double G = gvalue(g, NG, a.bx);
       if (gw == 0) {
-        v[j] = zscal(v[j], a.vcgl[l * R + j]);
+        v[j] = zscal(v[j], vcg[j]);
       } else {
         const double p2 = zabs2(v[j]);
         pEk += 0.5 * (G * G + 2.0 * G * kAo + kAo2) * p2;
@@ -238,11 +271,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && a.obs != nullptr;
   double pEx = 0.0;
   if (tid < NG) {
-    const size_t o = (size_t)NG * row + tid;
-    const zd wv = a.w[o];
     const zd vf = sVF[tid];
     zd h = sTW[tid];
-    h = zadd(h, zscal(wv, sVH[tid] + a.vion[tid]));
+    h = zadd(h, zscal(wv, sVH[tid] + vionv));
     h = zadd(h, vf);
     pEx = wv.x * vf.x + wv.y * vf.y;
     if (a.mode == M_HPSI) {
@@ -252,18 +283,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       const double hdt = 0.5 * a.dt;
       if (a.mode == M_S1) {
         a.ksum[o] = kk;
-        const zd u0 = a.u0[o];
-        a.unext[o] = mk(u0.x + hdt * kk.x, u0.y + hdt * kk.y);
+        a.unext[o] = mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y);
       } else if (a.mode == M_S2 || a.mode == M_S3) {
-        const zd ks = a.ksum[o];
-        a.ksum[o] = mk(ks.x + 2.0 * kk.x, ks.y + 2.0 * kk.y);
-        const zd u0 = a.u0[o];
+        a.ksum[o] = mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y);
         const double c = (a.mode == M_S2) ? hdt : a.dt;
-        a.unext[o] = mk(u0.x + c * kk.x, u0.y + c * kk.y);
+        a.unext[o] = mk(u0v.x + c * kk.x, u0v.y + c * kk.y);
       } else {
-        const zd ks = a.ksum[o];
-        const zd u0 = a.u0[o];
-        a.u0[o] = mk(u0.x + (ks.x + kk.x) * a.dt / 6.0, u0.y + (ks.y + kk.y) * a.dt / 6.0);
+        a.u0[o] = mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0);
       }
     }
   }
@@ -299,6 +325,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       }
     }
   }
+}
+
+// Twiddle tables of GroupFFT<NG> (2*NG complex).
+template <int NG>
+__global__ void twiddle_kernel(zd* tw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * NG) tw[i] = GroupFFT<NG>::table_value(i);
 }
 
 // Sums the per-row partials of each observables slot into

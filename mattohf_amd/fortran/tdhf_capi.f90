@@ -1,0 +1,135 @@
+!------------------------------------------------------------------------------
+! tdhf_capi.f90 -- iso_c_binding interface to libtdhf.so (include/tdhf.h).
+!
+! Lets a Fortran caller that holds ubk(NG,NB,NK) complex(8) -- the layout of
+! the reference's GS_calculation / RT_calculation (src/GS_calculation.f90:8,
+! src/RT_calculation.f90:10) -- call the gfx950 library directly.
+!------------------------------------------------------------------------------
+module tdhf_capi
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: TDHF_OK = 0
+
+  type, bind(C) :: tdhf_params
+    integer(c_int) :: ng, nk, nb, nbocc
+    real(c_double) :: acell, dt
+  end type tdhf_params
+
+  interface
+    integer(c_int) function tdhf_create(p, vion, vcg, vcgkk, ctx) bind(C, name="tdhf_create")
+      import :: c_int, c_double, c_ptr, tdhf_params
+      type(tdhf_params), intent(in) :: p
+      real(c_double), intent(in) :: vion(*)
+      type(c_ptr), value :: vcg, vcgkk
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function tdhf_destroy(ctx) bind(C, name="tdhf_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    type(c_ptr) function tdhf_last_error_c(ctx) bind(C, name="tdhf_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function tdhf_ion_potential(p, nion, z, alpha, rion, vion) &
+         bind(C, name="tdhf_ion_potential")
+      import :: c_int, c_double, tdhf_params
+      type(tdhf_params), intent(in) :: p
+      integer(c_int), value :: nion
+      real(c_double), intent(in) :: z(*), alpha(*), rion(*)
+      real(c_double), intent(out) :: vion(*)
+    end function
+
+    integer(c_int) function tdhf_get_tables(ctx, vcg, vq) bind(C, name="tdhf_get_tables")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: vcg(*), vq(*)
+    end function
+
+    integer(c_int) function tdhf_hpsi(ctx, u, w, A, hw) bind(C, name="tdhf_hpsi")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: u(*), w(*)
+      real(c_double), value :: A
+      complex(c_double_complex), intent(out) :: hw(*)
+    end function
+
+    integer(c_int) function tdhf_observables(ctx, u, A, etot, ecomps, nele, je) &
+         bind(C, name="tdhf_observables")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: u(*)
+      real(c_double), value :: A
+      real(c_double), intent(out) :: etot, ecomps(4), nele, je
+    end function
+
+    integer(c_int) function tdhf_rk4_step(ctx, u, Aave, A) bind(C, name="tdhf_rk4_step")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      real(c_double), value :: Aave, A
+    end function
+
+    integer(c_int) function tdhf_rt_run(ctx, u, nt, A, Aave, etot, nele, jtot, init3) &
+         bind(C, name="tdhf_rt_run")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(*), Aave(*)
+      real(c_double), intent(out) :: etot(*), nele(*), jtot(*), init3(3)
+    end function
+
+    integer(c_int) function tdhf_gs_run(ctx, u, nscf, ncg, tol, etot_hist, epsbk, epserr, &
+         niter) bind(C, name="tdhf_gs_run")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      integer(c_int), value :: nscf, ncg
+      real(c_double), value :: tol
+      real(c_double), intent(out) :: etot_hist(*), epsbk(*), epserr(*)
+      integer(c_int), intent(out) :: niter
+    end function
+
+    integer(c_int) function tdhf_gram_schmidt(ctx, u) bind(C, name="tdhf_gram_schmidt")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+    end function
+  end interface
+
+contains
+
+  ! Message of the last failed call (tdhf_last_error) as a Fortran string.
+  function tdhf_last_error(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ''
+    p = tdhf_last_error_c(ctx)
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do i = 1, 512
+      if (s(i) == c_null_char) exit
+      msg(i:i) = s(i)
+    end do
+  end function tdhf_last_error
+
+  ! Stops with the library's message when rc /= 0.
+  subroutine tdhf_check(rc, ctx, what)
+    integer(c_int), intent(in) :: rc
+    type(c_ptr), intent(in) :: ctx
+    character(len=*), intent(in) :: what
+    if (rc /= TDHF_OK) then
+      write(*,'(a,a,i4,a,a)') trim(what), ' failed (', rc, '): ', trim(tdhf_last_error(ctx))
+      stop 1
+    end if
+  end subroutine tdhf_check
+
+end module tdhf_capi
