@@ -110,6 +110,70 @@ def copy_probe(nbytes=1 << 30, reps=4):
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def batch_pulses(ntraj, nt, dt, T, seed):
+    """Seeded pulse set of the batched workload: A0 log-uniform in [0.01, 0.5],
+    omega uniform in [0.03, 0.10] Ha, CEP uniform in [0, 2 pi] (splitmix-seeded
+    numpy generator), A(t) = A0 sin^2(pi t/T) sin(omega t + CEP)."""
+    from mattohf_amd import tdhf as T_
+
+    rng = np.random.default_rng(seed)
+    A0 = np.exp(rng.uniform(math.log(0.01), math.log(0.5), ntraj))
+    om = rng.uniform(0.03, 0.10, ntraj)
+    cep = rng.uniform(0.0, 2 * math.pi, ntraj)
+    A = np.empty((ntraj, nt))
+    Aave = np.empty((ntraj, nt))
+    for t in range(ntraj):
+        oc, tp, e0, ph, ne = T_.pulse_lp_params(A0[t], om[t], T)
+        _, A[t], Aave[t], _ = T_.define_afield_lp(nt, dt, oc, tp, e0, ph + cep[t] / (2 * math.pi), ne)
+    return A, Aave
+
+
+def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
+    """Batched run of len(nks) trajectories (NG, NB, a, Z of cfg; NK per
+    trajectory).  Each trajectory starts from the ground state of its NK."""
+    from mattohf_amd import tdhf as T
+
+    ng, nb, a = cfg["ng"], cfg["nb"], cfg["acell"]
+    gs, vi = {}, {}
+    for nk in sorted(set(nks)):
+        vion = T.ion_potential(ng, nk, a, [cfg["z"]], [cfg["alpha"]], [0.0])
+        with T.TDHF(ng, nk, nb, cfg["nbocc"], a, cfg["dt"], vion) as G:
+            u = G.Gram_Schmidt(T.init_wf(ng, nb, nk, a))
+            gs[nk] = G.GS_calculation(u, gs_iters, 4, tol=1e-11)[0]
+        vi[nk] = vion
+    ntraj = len(nks)
+    A, Aave = batch_pulses(ntraj, warmup + steps + 25, cfg["dt"], cfg["T"], seed)
+    B = T.Batch(ng, nb, cfg["nbocc"], a, cfg["dt"], nks, np.array([vi[k] for k in nks]))
+    B.upload([gs[k] for k in nks])
+    if autotune:
+        B.autotune(A[:, :25], Aave[:, :25])
+    if warmup:
+        B.propagate(A[:, :warmup], Aave[:, :warmup])
+    t0 = time.perf_counter()
+    e, n, j, _ = B.propagate(A[:, warmup:warmup + steps], Aave[:, warmup:warmup + steps])
+    wall = time.perf_counter() - t0
+    B.set_kernel_timing(True)
+    B.propagate(A[:, warmup + steps:warmup + steps + 10], Aave[:, warmup + steps:warmup + steps + 10])
+    launches, ms = B.kernel_stats()
+    B.set_kernel_timing(False)
+    waves = B.waves()
+    B.close()
+    f_launch = sum(flops_per_step(ng, k, nb, cfg["nbocc"]) for k in nks) / 4.0
+    avg_s = ms / launches / 1e3
+    return {
+        "traj_steps_per_s": ntraj * steps / wall,
+        "ntraj": ntraj,
+        "nk_counts": {str(k): int(sum(1 for x in nks if x == k)) for k in sorted(set(nks))},
+        "steps": steps,
+        "ms_per_batch_step": wall / steps * 1e3,
+        "waves_per_workgroup": waves,
+        "avg_launch_us": avg_s * 1e6,
+        "achieved_tflops": f_launch / avg_s / 1e12,
+        "frac": f_launch / avg_s / 1e12 / PEAK_FP64_TFLOPS,
+        "observables_finite": bool(np.isfinite(e).all() and np.isfinite(j).all()),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +182,9 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=25)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-traj", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -194,9 +261,25 @@ def main():
         },
         "cpu_baseline": None,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(cfg, vion, G, u, A, Aave, budget_s=args.cpu_seconds)
     G.close()
+    if not args.no_batch:
+        rng = np.random.default_rng(20261202)
+        mixed = [int(k) for k in rng.choice([16, 32, 64], size=args.batch_traj)]
+        bm = batch_bench(cfg, mixed, args.batch_steps, 5, seed=20261202)
+        bu = batch_bench(cfg, [64] * max(1, args.batch_traj // 4), args.batch_steps, 5,
+                         seed=20261203)
+        out["batched"] = {
+            "workload": "independent trajectories, one launch per RK4 stage for all, "
+                        "hipGraph-replayed steps, autotuned waves/workgroup",
+            "seed": 20261202,
+            "mixed_nk": bm,
+            "uniform_nk64": bu,
+            "batched_vs_plain_at_nk64": bu["traj_steps_per_s"] / value,
+        }
+    if not args.no_cpu:
+        G2 = setup(cfg)[0]
+        out["cpu_baseline"] = cpu_baseline(cfg, vion, G2, u, A, Aave, budget_s=args.cpu_seconds)
+        G2.close()
     print(json.dumps(out), flush=True)
 
 

@@ -138,6 +138,47 @@ int tdhf_gram_schmidt(tdhf_ctx* ctx, double* u);
 int tdhf_set_kernel_timing(tdhf_ctx* ctx, int enable);
 int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);
 
+/* ------------------------------------------------------------------------
+ * Batched trajectories.  No counterpart in the reference (which runs one
+ * trajectory per process): many independent RT runs of one cell (NG, NB,
+ * NBocc, acell, dt shared; NK, ion potential and field per trajectory)
+ * advance together, one launch per RK4 stage for all of them, the steps
+ * replayed from a captured hipGraph.  Each trajectory is exactly the run
+ * tdhf_rt_run would make with its own context.
+ *   nk[t]   : k-points of trajectory t (p->nk is ignored)
+ *   vion    : NG*ntraj doubles, trajectory t at vion + NG*t
+ *   u       : all orbitals concatenated; trajectory t is (NG,NB,nk[t])
+ *             complex starting after those of trajectories 0..t-1
+ *   A, Aave : ntraj*nt doubles, trajectory t at offset nt*t
+ *   etot, nele, jtot : ntraj*nt (may be NULL); init3 : ntraj*3 (may be NULL)
+ * ------------------------------------------------------------------------ */
+typedef struct tdhf_batch tdhf_batch;
+
+int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const double* vion,
+                      tdhf_batch** out);
+int tdhf_batch_destroy(tdhf_batch* b);
+const char* tdhf_batch_last_error(const tdhf_batch* b);
+/* traj >= 0: one trajectory's orbitals; traj < 0: all, concatenated. */
+int tdhf_batch_upload(tdhf_batch* b, int traj, const double* u);
+int tdhf_batch_download(tdhf_batch* b, int traj, double* u);
+/* nt RK4 steps of every trajectory with the observables after each step. */
+int tdhf_batch_propagate(tdhf_batch* b, int nt, const double* A, const double* Aave,
+                         double* etot, double* nele, double* jtot, double* init3);
+/* create + upload + propagate + download + destroy in one call. */
+int tdhf_batch_run(const tdhf_params* p, int ntraj, const int* nk, const double* vion,
+                   double* u, int nt, const double* A, const double* Aave, double* etot,
+                   double* nele, double* jtot, double* init3);
+/* Launch shape: waves per workgroup of the fused kernel (2, 4 or 8).
+ * tdhf_batch_autotune times nt_trial steps for each candidate on a copy of the
+ * state (the batch state is unchanged), keeps the fastest and remembers it for
+ * batches of the same shape created later in the process. */
+int tdhf_batch_set_waves(tdhf_batch* b, int nw);
+int tdhf_batch_get_waves(const tdhf_batch* b);
+int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
+                        int* best_nw);
+int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);
+int tdhf_batch_kernel_stats(tdhf_batch* b, long* launches, double* total_ms);
+
 #ifdef __cplusplus
 }
 #endif

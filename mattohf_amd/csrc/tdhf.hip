@@ -773,3 +773,6 @@ int tdhf_kernel_stats(tdhf_ctx* c, long* launches, double* total_ms) {
 }
 
 }  // extern "C"
+
+// batched trajectories (tdhf_batch_*)
+#include "tdhf_batch_impl.h"

@@ -1,0 +1,550 @@
+// tdhf_batch_impl.h -- batched propagation of independent trajectories
+// (included by tdhf.hip after the single-trajectory helpers).
+//
+// Every trajectory has its own k-mesh (NK may differ), ion potential, field
+// A(t) and RK4 buffers; NG, NB, NBocc, acell and dt are shared.  One launch of
+// apply_kernel per RK4 stage covers the rows (k,i) of all trajectories: the
+// schedule (rowmap) lists rows longest-first within each XCD, and pins each
+// trajectory to one XCD (workgroups are dispatched round-robin over the 8
+// XCDs, so entry b runs on XCD b % 8) so that its orbitals stay in that XCD's
+// L2.  The RK4 steps are captured as a hipGraph and replayed; the launch
+// shape (waves per workgroup) is picked by tdhf_batch_autotune.
+#pragma once
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <numeric>
+
+struct tdhf_batch {
+  tdhf_params p{};
+  int ntraj = 0;
+  std::vector<int> nk;
+  std::string err = "no error";
+  hipStream_t stream = nullptr;
+  std::vector<tdhf_ctx*> tabs;  // one context (tables) per distinct NK
+  std::vector<int> tab_of;
+  std::vector<size_t> off;      // state offset (complex elements) per trajectory
+  size_t total = 0;             // complex elements of all trajectories
+  double* d_vion = nullptr;
+  zd* d_state = nullptr;        // 4 * total: u0 | ksum | sa | sb
+  tdhf::TrajDesc* d_td = nullptr;
+  std::vector<tdhf::TrajDesc> h_td;
+  int2* d_rowmap = nullptr;
+  int ngrid = 0;
+  double *d_A = nullptr, *d_Aave = nullptr;
+  long A_cap = 0;
+  double* d_obs = nullptr;
+  long obs_cap = 0;
+  std::vector<size_t> obs_off;  // doubles per trajectory per slot: stride_t
+  double* d_res = nullptr;
+  long res_cap = 0;
+  int* d_step = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+  int graph_nw = 0;
+  int nw = 4;
+  bool timing = false;
+  long launches = 0;
+  double total_ms = 0.0;
+  std::vector<hipEvent_t> ev;
+  size_t ev_used = 0;
+};
+
+namespace {
+
+int bfail(tdhf_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  g_err = buf;
+  return code;
+}
+
+#define BHIP(b, expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return bfail((b), TDHF_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                   __FILE__, __LINE__);                                                       \
+  } while (0)
+
+template <typename T>
+int balloc(tdhf_batch* b, T** p, size_t n) {
+  if (*p) return 0;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(T) + 16);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return bfail(b, TDHF_ENOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                 hipGetErrorString(e));
+  }
+  return 0;
+}
+
+size_t bstate(const tdhf_batch* b, int t) { return (size_t)b->p.ng * b->p.nb * b->nk[t]; }
+
+int batch_valid_nw(const tdhf_batch* b, int nw) {
+  return nw >= 1 && nw <= 8 && 64 * nw >= b->p.ng;
+}
+
+// Schedule: LPT over XCDs by trajectory cost, rows longest-first per XCD,
+// interleaved so that entry b goes to XCD b % 8.
+void build_schedule(const tdhf_batch* b, std::vector<int2>& map) {
+  constexpr int kXcd = 8;
+  const int nb = b->p.nb;
+  std::vector<int> order(b->ntraj);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int t) { return (double)b->nk[t] * b->nk[t] * nb * b->p.nbocc + b->nk[t] * nb; };
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  std::vector<double> load(kXcd, 0.0);
+  std::vector<std::vector<int>> bins(kXcd);
+  for (int t : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost(t);
+    bins[x].push_back(t);  // already in decreasing cost order
+  }
+  std::vector<std::vector<int2>> q(kXcd);
+  size_t longest = 0;
+  for (int x = 0; x < kXcd; ++x) {
+    for (int t : bins[x])
+      for (int r = 0; r < nb * b->nk[t]; ++r) q[x].push_back(make_int2(t, r));
+    longest = std::max(longest, q[x].size());
+  }
+  map.clear();
+  for (size_t i = 0; i < longest; ++i)
+    for (int x = 0; x < kXcd; ++x) map.push_back(i < q[x].size() ? q[x][i] : make_int2(-1, 0));
+  while (!map.empty() && map.back().x < 0) map.pop_back();
+}
+
+int batch_update_desc(tdhf_batch* b) {
+  for (int t = 0; t < b->ntraj; ++t) {
+    tdhf::TrajDesc& d = b->h_td[t];
+    tdhf_ctx* c = b->tabs[b->tab_of[t]];
+    d.nk = b->nk[t];
+    d.obs_stride = 3 * b->p.nb * b->nk[t] + 3;
+    d.occ = 2.0 / (double)b->nk[t];
+    d.kx = c->d_kx;
+    d.vion = b->d_vion + (size_t)b->p.ng * t;
+    d.vcgl = c->d_vcgl;
+    d.vtl = c->d_vtl;
+    d.u0 = b->d_state + b->off[t];
+    d.ksum = b->d_state + b->total + b->off[t];
+    d.sa = b->d_state + 2 * b->total + b->off[t];
+    d.sb = b->d_state + 3 * b->total + b->off[t];
+    d.A = b->d_A ? b->d_A + (size_t)b->A_cap * t : nullptr;
+    d.Aave = b->d_Aave ? b->d_Aave + (size_t)b->A_cap * t : nullptr;
+    d.obs = b->d_obs ? b->d_obs + b->obs_off[t] * (size_t)b->obs_cap : nullptr;
+  }
+  BHIP(b, hipMemcpyAsync(b->d_td, b->h_td.data(), b->h_td.size() * sizeof(tdhf::TrajDesc),
+                         hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (b->timing) {
+    if (b->ev_used + 2 > b->ev.size())
+      for (int i = 0; i < 64; ++i) {
+        hipEvent_t e;
+        BHIP(b, hipEventCreate(&e));
+        b->ev.push_back(e);
+      }
+    e0 = b->ev[b->ev_used++];
+    e1 = b->ev[b->ev_used++];
+    BHIP(b, hipEventRecord(e0, b->stream));
+  }
+  const size_t lds = lds_for(b->p.ng, nw);
+  switch (b->p.ng) {
+    case 16: launch_apply_t<16>(a, b->ngrid, nw, lds, b->stream); break;
+    case 32: launch_apply_t<32>(a, b->ngrid, nw, lds, b->stream); break;
+    case 64: launch_apply_t<64>(a, b->ngrid, nw, lds, b->stream); break;
+    case 128: launch_apply_t<128>(a, b->ngrid, nw, lds, b->stream); break;
+  }
+  BHIP(b, hipGetLastError());
+  if (b->timing) BHIP(b, hipEventRecord(e1, b->stream));
+  return 0;
+}
+
+int batch_harvest(tdhf_batch* b) {
+  BHIP(b, hipStreamSynchronize(b->stream));
+  if (!b->timing) return 0;
+  for (size_t i = 0; i + 1 < b->ev_used; i += 2) {
+    float ms = 0.f;
+    BHIP(b, hipEventElapsedTime(&ms, b->ev[i], b->ev[i + 1]));
+    b->total_ms += ms;
+    b->launches += 1;
+  }
+  b->ev_used = 0;
+  return 0;
+}
+
+tdhf::ApplyArgs batch_args(const tdhf_batch* b) {
+  tdhf::ApplyArgs a{};
+  a.nb = b->p.nb;
+  a.nbocc = b->p.nbocc;
+  a.nbw = b->p.nb;
+  a.bx = 2.0 * (4.0 * std::atan(1.0)) / b->p.acell;
+  a.dv = std::fabs(b->p.acell) / (double)b->p.ng;
+  a.dt = b->p.dt;
+  a.tw = b->tabs[0]->d_tw;
+  a.td = b->d_td;
+  a.rowmap = b->d_rowmap;
+  a.step_base = b->d_step;
+  return a;
+}
+
+int batch_enqueue_steps(tdhf_batch* b, int n, int nw) {
+  tdhf::ApplyArgs a = batch_args(b);
+  for (int j = 0; j < n; ++j) {
+    a.jstep = j;
+    for (int m = tdhf::M_S1; m <= tdhf::M_S4; ++m) {
+      a.mode = m;
+      CHKRET(batch_launch(b, a, nw));
+    }
+  }
+  return 0;
+}
+
+int batch_run_steps(tdhf_batch* b, int nt) {
+  int done = 0;
+  if (!b->timing && nt >= kGraphSteps) {
+    if (b->gexec && b->graph_nw != b->nw) {
+      (void)hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+    }
+    if (!b->gexec) {
+      hipGraph_t g = nullptr;
+      BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_enqueue_steps(b, kGraphSteps, b->nw);
+      if (rc == 0)
+        hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, b->stream, b->d_step,
+                           kGraphSteps);
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return bfail(b, TDHF_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+      e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return bfail(b, TDHF_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+      b->graph_steps = kGraphSteps;
+      b->graph_nw = b->nw;
+    }
+    while (nt - done >= b->graph_steps) {
+      BHIP(b, hipGraphLaunch(b->gexec, b->stream));
+      done += b->graph_steps;
+    }
+  }
+  if (done < nt) {
+    CHKRET(batch_enqueue_steps(b, nt - done, b->nw));
+    hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, b->stream, b->d_step,
+                       nt - done);
+    BHIP(b, hipGetLastError());
+  }
+  return 0;
+}
+
+int batch_ensure(tdhf_batch* b, long nt) {
+  if (nt > b->A_cap || nt + 1 > b->obs_cap) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    dfree(b->d_A);
+    dfree(b->d_Aave);
+    dfree(b->d_obs);
+    dfree(b->d_res);
+    const long cap = std::max(nt, 1L);
+    CHKRET(balloc(b, &b->d_A, (size_t)cap * b->ntraj));
+    CHKRET(balloc(b, &b->d_Aave, (size_t)cap * b->ntraj));
+    size_t per_slot = 0;
+    for (int t = 0; t < b->ntraj; ++t) per_slot += 3 * (size_t)b->p.nb * b->nk[t] + 3;
+    CHKRET(balloc(b, &b->d_obs, per_slot * (size_t)(cap + 1)));
+    CHKRET(balloc(b, &b->d_res, (size_t)7 * b->ntraj * (cap + 1)));
+    b->A_cap = cap;
+    b->obs_cap = cap + 1;
+    b->res_cap = cap + 1;
+    CHKRET(batch_update_desc(b));
+  }
+  return 0;
+}
+
+void batch_release(tdhf_batch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  for (auto e : b->ev) (void)hipEventDestroy(e);
+  b->ev.clear();
+  dfree(b->d_vion); dfree(b->d_state); dfree(b->d_td); dfree(b->d_rowmap);
+  dfree(b->d_A); dfree(b->d_Aave); dfree(b->d_obs); dfree(b->d_res); dfree(b->d_step);
+  for (auto c : b->tabs) tdhf_destroy(c);
+  b->tabs.clear();
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  b->stream = nullptr;
+}
+
+// autotuner cache: shape -> waves per workgroup
+std::map<std::vector<int>, int>& tune_cache() {
+  static std::map<std::vector<int>, int> m;
+  return m;
+}
+
+std::vector<int> tune_key(const tdhf_batch* b) {
+  std::vector<int> k = {b->p.ng, b->p.nb, b->p.nbocc};
+  std::vector<int> n = b->nk;
+  std::sort(n.begin(), n.end());
+  k.insert(k.end(), n.begin(), n.end());
+  return k;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tdhf_batch_last_error(const tdhf_batch* b) { return b ? b->err.c_str() : g_err.c_str(); }
+
+int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const double* vion,
+                      tdhf_batch** out) {
+  if (!out) return bfail(nullptr, TDHF_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (!p || ntraj < 1 || !nk || !vion) return bfail(nullptr, TDHF_EINVAL, "invalid batch arguments");
+  tdhf_params q = *p;
+  for (int t = 0; t < ntraj; ++t) {
+    q.nk = nk[t];
+    CHKRET(valid_params(&q, nullptr));
+  }
+  tdhf_batch* b = new tdhf_batch();
+  b->p = *p;
+  b->ntraj = ntraj;
+  b->nk.assign(nk, nk + ntraj);
+  auto bail = [&](int rc) {
+    g_err = b->err;
+    batch_release(b);
+    delete b;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    bfail(b, TDHF_EDEVICE, "hipStreamCreate failed");
+    return bail(TDHF_EDEVICE);
+  }
+  // tables: one context per distinct NK (its ion potential is not used)
+  std::map<int, int> idx;
+  b->tab_of.resize(ntraj);
+  for (int t = 0; t < ntraj; ++t) {
+    auto it = idx.find(nk[t]);
+    if (it == idx.end()) {
+      q.nk = nk[t];
+      tdhf_ctx* c = nullptr;
+      int rc = tdhf_create(&q, vion + (size_t)p->ng * t, nullptr, nullptr, &c);
+      if (rc) {
+        b->err = g_err;
+        return bail(rc);
+      }
+      it = idx.emplace(nk[t], (int)b->tabs.size()).first;
+      b->tabs.push_back(c);
+    }
+    b->tab_of[t] = it->second;
+  }
+  b->off.resize(ntraj);
+  b->obs_off.resize(ntraj);
+  size_t acc = 0, oacc = 0;
+  for (int t = 0; t < ntraj; ++t) {
+    b->off[t] = acc;
+    acc += bstate(b, t);
+    b->obs_off[t] = oacc;
+    oacc += 3 * (size_t)p->nb * nk[t] + 3;
+  }
+  b->total = acc;
+  int rc;
+  if ((rc = balloc(b, &b->d_vion, (size_t)p->ng * ntraj)) || (rc = balloc(b, &b->d_state, 4 * acc)) ||
+      (rc = balloc(b, &b->d_td, (size_t)ntraj)) || (rc = balloc(b, &b->d_step, 1)))
+    return bail(rc);
+  if (hipMemcpy(b->d_vion, vion, sizeof(double) * p->ng * ntraj, hipMemcpyHostToDevice) != hipSuccess) {
+    bfail(b, TDHF_EDEVICE, "vion upload failed");
+    return bail(TDHF_EDEVICE);
+  }
+  std::vector<int2> map;
+  build_schedule(b, map);
+  b->ngrid = (int)map.size();
+  if ((rc = balloc(b, &b->d_rowmap, map.size()))) return bail(rc);
+  if (hipMemcpy(b->d_rowmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    bfail(b, TDHF_EDEVICE, "schedule upload failed");
+    return bail(TDHF_EDEVICE);
+  }
+  b->h_td.resize(ntraj);
+  if ((rc = batch_update_desc(b))) return bail(rc);
+  // launch shape: cached autotuner choice, else 4 waves per workgroup
+  auto ct = tune_cache().find(tune_key(b));
+  b->nw = ct != tune_cache().end() ? ct->second : std::max(4, (p->ng + 63) / 64);
+  {
+    const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
+                     : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
+                     : p->ng == 64 ? (const void*)tdhf::apply_kernel<64>
+                                   : (const void*)tdhf::apply_kernel<128>;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)lds_for(p->ng, 8));
+  }
+  *out = b;
+  return 0;
+}
+
+int tdhf_batch_destroy(tdhf_batch* b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_release(b);
+  delete b;
+  return 0;
+}
+
+int tdhf_batch_upload(tdhf_batch* b, int traj, const double* u) {
+  if (!b || !u || traj >= b->ntraj) return bfail(b, TDHF_EINVAL, "invalid upload arguments");
+  if (traj < 0)
+    BHIP(b, hipMemcpyAsync(b->d_state, u, b->total * sizeof(zd), hipMemcpyHostToDevice, b->stream));
+  else
+    BHIP(b, hipMemcpyAsync(b->d_state + b->off[traj], u, bstate(b, traj) * sizeof(zd),
+                           hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int tdhf_batch_download(tdhf_batch* b, int traj, double* u) {
+  if (!b || !u || traj >= b->ntraj) return bfail(b, TDHF_EINVAL, "invalid download arguments");
+  if (traj < 0)
+    BHIP(b, hipMemcpyAsync(u, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToHost, b->stream));
+  else
+    BHIP(b, hipMemcpyAsync(u, b->d_state + b->off[traj], bstate(b, traj) * sizeof(zd),
+                           hipMemcpyDeviceToHost, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int tdhf_batch_propagate(tdhf_batch* b, int nt, const double* A, const double* Aave, double* etot,
+                         double* nele, double* jtot, double* init3) {
+  if (!b) return bfail(nullptr, TDHF_EINVAL, "batch is NULL");
+  if (nt < 1 || !A || !Aave) return bfail(b, TDHF_EINVAL, "invalid field arrays");
+  CHKRET(batch_ensure(b, nt));
+  for (int t = 0; t < b->ntraj; ++t) {
+    BHIP(b, hipMemcpyAsync(b->d_A + (size_t)b->A_cap * t, A + (size_t)nt * t, nt * sizeof(double),
+                           hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipMemcpyAsync(b->d_Aave + (size_t)b->A_cap * t, Aave + (size_t)nt * t,
+                           nt * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  }
+  const int zero = 0;
+  BHIP(b, hipMemcpyAsync(b->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  CHKRET(batch_run_steps(b, nt));
+  // observables of the final states (slot nt)
+  tdhf::ApplyArgs a = batch_args(b);
+  a.mode = tdhf::M_OBS;
+  a.jstep = -1;
+  CHKRET(batch_launch(b, a, b->nw));
+  const int ns = nt + 1;
+  const int n = b->ntraj * ns;
+  hipLaunchKernelGGL(tdhf::obs_reduce_batch_kernel, dim3((n + 63) / 64), dim3(64), 0, b->stream,
+                     b->d_td, b->ntraj, b->p.nb, ns, std::fabs(b->p.acell) / (double)b->p.ng,
+                     std::fabs(b->p.acell), b->d_res);
+  BHIP(b, hipGetLastError());
+  std::vector<double> res((size_t)7 * n);
+  BHIP(b, hipMemcpyAsync(res.data(), b->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         b->stream));
+  CHKRET(batch_harvest(b));
+  for (int t = 0; t < b->ntraj; ++t) {
+    const double* r0 = &res[(size_t)7 * ns * t];
+    if (init3) {
+      init3[3 * t] = r0[0];
+      init3[3 * t + 1] = r0[1];
+      init3[3 * t + 2] = r0[2];
+    }
+    for (int it = 0; it < nt; ++it) {
+      const double* r = r0 + (size_t)7 * (it + 1);
+      if (etot) etot[(size_t)nt * t + it] = r[0];
+      if (nele) nele[(size_t)nt * t + it] = r[1];
+      if (jtot) jtot[(size_t)nt * t + it] = r[2];
+    }
+  }
+  return 0;
+}
+
+int tdhf_batch_run(const tdhf_params* p, int ntraj, const int* nk, const double* vion, double* u,
+                   int nt, const double* A, const double* Aave, double* etot, double* nele,
+                   double* jtot, double* init3) {
+  tdhf_batch* b = nullptr;
+  CHKRET(tdhf_batch_create(p, ntraj, nk, vion, &b));
+  int rc = tdhf_batch_upload(b, -1, u);
+  if (!rc) rc = tdhf_batch_propagate(b, nt, A, Aave, etot, nele, jtot, init3);
+  if (!rc) rc = tdhf_batch_download(b, -1, u);
+  if (rc) g_err = b->err;
+  tdhf_batch_destroy(b);
+  return rc;
+}
+
+int tdhf_batch_set_waves(tdhf_batch* b, int nw) {
+  if (!b || !batch_valid_nw(b, nw)) return bfail(b, TDHF_EINVAL, "invalid waves per workgroup %d", nw);
+  b->nw = nw;
+  return 0;
+}
+
+int tdhf_batch_get_waves(const tdhf_batch* b) { return b ? b->nw : 0; }
+
+int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
+                        int* best_nw) {
+  if (!b || nt_trial < 1 || !A || !Aave) return bfail(b, TDHF_EINVAL, "invalid autotune arguments");
+  // keep the state: the trial steps run on a copy
+  zd* save = nullptr;
+  CHKRET(balloc(b, &save, b->total));
+  BHIP(b, hipMemcpyAsync(save, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
+  int best = b->nw;
+  double best_t = 1e300;
+  const int cand[3] = {2, 4, 8};
+  for (int nw : cand) {
+    if (!batch_valid_nw(b, nw)) continue;
+    b->nw = nw;
+    int rc = 0;
+    double dt_ms = 0.0;
+    for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up (graph capture)
+      BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
+      BHIP(b, hipStreamSynchronize(b->stream));
+      auto t0 = std::chrono::steady_clock::now();
+      rc = tdhf_batch_propagate(b, nt_trial, A, Aave, nullptr, nullptr, nullptr, nullptr);
+      auto t1 = std::chrono::steady_clock::now();
+      dt_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    }
+    if (rc) {
+      dfree(save);
+      return rc;
+    }
+    if (dt_ms < best_t) {
+      best_t = dt_ms;
+      best = nw;
+    }
+  }
+  BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  dfree(save);
+  b->nw = best;
+  tune_cache()[tune_key(b)] = best;
+  if (best_nw) *best_nw = best;
+  return 0;
+}
+
+int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable) {
+  if (!b) return bfail(nullptr, TDHF_EINVAL, "batch is NULL");
+  CHKRET(batch_harvest(b));
+  b->timing = enable != 0;
+  b->launches = 0;
+  b->total_ms = 0.0;
+  return 0;
+}
+
+int tdhf_batch_kernel_stats(tdhf_batch* b, long* launches, double* total_ms) {
+  if (!b) return bfail(nullptr, TDHF_EINVAL, "batch is NULL");
+  CHKRET(batch_harvest(b));
+  if (launches) *launches = b->launches;
+  if (total_ms) *total_ms = b->total_ms;
+  return 0;
+}
+
+}  // extern "C"

@@ -30,6 +30,25 @@ enum ApplyMode : int {
   M_OBS = 5    // observables only
 };
 
+// One trajectory of a batch: its k-mesh, tables, RK4 buffers, field and
+// observables slots.  The batched launch covers the rows of all trajectories.
+struct TrajDesc {
+  int nk;
+  int obs_stride;           // 3*NB*nk + 3
+  double occ;               // 2/nk
+  const double* kx;
+  const double* vion;
+  const double* vcgl;
+  const double* vtl;
+  zd* u0;
+  zd* ksum;
+  zd* sa;
+  zd* sb;
+  const double* A;
+  const double* Aave;
+  double* obs;
+};
+
 // Offsets inside one observables slot: rows*3 partials then (N, Eion, EH).
 struct ApplyArgs {
   int nk, nb, nbocc, nbw;  // nbw = columns of w / out per k-point
@@ -54,6 +73,8 @@ struct ApplyArgs {
   zd* unext;                // RK4: next stage input
   double* obs;              // observables slots, or nullptr
   int mode;
+  const TrajDesc* td;       // batched launch: trajectory table, or nullptr
+  const int2* rowmap;       // batched launch: (trajectory, row) per workgroup
 };
 
 __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
@@ -80,7 +101,41 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const int nw = blockDim.x >> 6;
   const int l = lane % L, gw = lane / L;
   const int grp = wave * GPW + gw, ngrp = nw * GPW;
-  const int row = blockIdx.x;
+  // per-workgroup operands: from the arguments, or from the trajectory table
+  int row = blockIdx.x;
+  int nk = a.nk, obs_stride = a.obs_stride;
+  double occ = a.occ;
+  const double *kx = a.kx, *vion = a.vion, *vcgl = a.vcgl, *vtl = a.vtl;
+  const double *Aarr = a.Aarr, *Aavearr = a.Aavearr;
+  const zd *u = a.u, *w = a.w;
+  zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext;
+  double* obs = a.obs;
+  if (a.td) {
+    const int2 rm = a.rowmap[blockIdx.x];
+    if (rm.x < 0) return;  // padding entry of the schedule (whole workgroup)
+    const TrajDesc& t = a.td[rm.x];
+    row = rm.y;
+    nk = t.nk;
+    obs_stride = t.obs_stride;
+    occ = t.occ;
+    kx = t.kx;
+    vion = t.vion;
+    vcgl = t.vcgl;
+    vtl = t.vtl;
+    Aarr = t.A;
+    Aavearr = t.Aave;
+    u0 = t.u0;
+    ksum = t.ksum;
+    obs = t.obs;
+    switch (a.mode) {
+      case M_S1: u = t.u0; unext = t.sa; break;
+      case M_S2: u = t.sa; unext = t.sb; break;
+      case M_S3: u = t.sb; unext = t.sa; break;
+      case M_S4: u = t.sa; unext = nullptr; break;
+      default: u = t.u0; unext = nullptr; break;
+    }
+    w = u;
+  }
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
   zd* T = smem + grp * F::TSZ;
@@ -96,10 +151,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   double Akin = a.A, Aobs = a.Aobs;
   if (a.step_base) {
     step = *a.step_base + a.jstep;
-    const double At = a.Aarr[step], Aav = a.Aavearr[step];
+    const double At = Aarr[step], Aav = Aavearr[step];
     if (a.mode == M_S1) {
       Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
-      Aobs = step > 0 ? a.Aarr[step - 1] : 0.0;
+      Aobs = step > 0 ? Aarr[step - 1] : 0.0;
       slot = step;
     } else if (a.mode == M_S2 || a.mode == M_S3) {
       Akin = Aav;
@@ -117,15 +172,15 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   zd wv = mk(0.0, 0.0), u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0);
   double vionv = 0.0;
   if (tid < NG) {
-    wv = a.w[o];
+    wv = w[o];
     sW[tid] = wv;
-    vionv = a.vion[tid];
+    vionv = vion[tid];
     if (a.mode >= M_S1 && a.mode <= M_S4) {
-      u0v = a.u0[o];
-      if (a.mode != M_S1) ksv = a.ksum[o];
+      u0v = u0[o];
+      if (a.mode != M_S1) ksv = ksum[o];
     }
   }
-  const double kxv = a.kx[ik];
+  const double kxv = kx[ik];
   F fft;
   fft.init(a.tw, l);
 
@@ -139,10 +194,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
 
   // ---- exchange pair loop (ubkwbk_vFwbk, src/Functions.f90:247-257) ----
   // u(q,j) of the next pair is fetched while the current pair is transformed.
-  const int nitem = a.nk * a.nbocc;
+  const int nitem = nk * a.nbocc;
   auto fetch = [&](int item) {
     const int iq = item / a.nbocc, jb = item - iq * a.nbocc;
-    const zd* urow = a.u + (size_t)NG * (jb + a.nb * iq);
+    const zd* urow = u + (size_t)NG * (jb + a.nb * iq);
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       un[m] = urow[L * m + l];
@@ -163,7 +218,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       v[m] = zmulc(sW[L * m + l], uq[m]);
       den[m] += zabs2(uq[m]);
     });
-    const double* vt = a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + l * R;
+    const double* vt = vtl + (size_t)(ik - iq + nk - 1) * NG + l * R;
     double vtr[R];
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
@@ -188,7 +243,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   double vcg[R];
   static_for<0, R>([&](auto jc) {
     constexpr int j = decltype(jc)::value;
-    vcg[j] = (wave == 0 && gw == 0) ? a.vcgl[l * R + j] : 0.0;
+    vcg[j] = (wave == 0 && gw == 0) ? vcgl[l * R + j] : 0.0;
   });
 
   // ---- reduction over all groups through LDS (fixed order) ----
@@ -215,7 +270,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   for (int idx = tid; idx < NG; idx += blockDim.x) {
     double s = 0.0;
     for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
-    sRho[idx] = a.occ * s;
+    sRho[idx] = occ * s;
   }
   __syncthreads();
 
@@ -268,7 +323,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   __syncthreads();
 
   // ---- epilogue: H w and the mode-specific update ----
-  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && a.obs != nullptr;
+  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && obs != nullptr;
   double pEx = 0.0;
   if (tid < NG) {
     const zd vf = sVF[tid];
@@ -282,14 +337,14 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       const zd kk = mk(h.y, -h.x);  // -i * H w
       const double hdt = 0.5 * a.dt;
       if (a.mode == M_S1) {
-        a.ksum[o] = kk;
-        a.unext[o] = mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y);
+        ksum[o] = kk;
+        unext[o] = mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y);
       } else if (a.mode == M_S2 || a.mode == M_S3) {
-        a.ksum[o] = mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y);
+        ksum[o] = mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y);
         const double c = (a.mode == M_S2) ? hdt : a.dt;
-        a.unext[o] = mk(u0v.x + c * kk.x, u0v.y + c * kk.y);
+        unext[o] = mk(u0v.x + c * kk.x, u0v.y + c * kk.y);
       } else {
-        a.u0[o] = mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0);
+        u0[o] = mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0);
       }
     }
   }
@@ -306,17 +361,17 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     __syncthreads();
     if (tid == 0) {
       if (NG > 64) pEx += sRed[2];
-      const double occr = ib < a.nbocc ? a.occ : 0.0;
-      double* p = a.obs + (size_t)slot * a.obs_stride;
+      const double occr = ib < a.nbocc ? occ : 0.0;
+      double* p = obs + (size_t)slot * obs_stride;
       p[3 * row + 0] = occr * sRed[0] / (double)NG * a.dv;
       p[3 * row + 1] = occr * sRed[1] / (double)NG;
       p[3 * row + 2] = occr * pEx * a.dv;
       if (row == 0) {
-        const int rows = a.nbw * a.nk;
+        const int rows = a.nbw * nk;
         double n = 0.0, ei = 0.0, eh = 0.0;
         for (int x = 0; x < NG; ++x) {
           n += sRho[x];
-          ei += a.vion[x] * sRho[x];
+          ei += vion[x] * sRho[x];
           eh += sVH[x] * sRho[x];
         }
         p[3 * rows + 0] = n * a.dv;
@@ -350,6 +405,32 @@ __global__ void obs_reduce_kernel(const double* obs, int stride, int rows, int n
   ex = 0.5 * ex;
   const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
   double* o = out + (size_t)7 * s;
+  o[0] = ek + ei + eh + ex;
+  o[1] = n;
+  o[2] = -(jj * dv / vcell);
+  o[3] = ek;
+  o[4] = ei;
+  o[5] = eh;
+  o[6] = ex;
+}
+
+// Batched variant: one thread per (trajectory, slot); out[(t*nslots + s)*7].
+__global__ void obs_reduce_batch_kernel(const TrajDesc* td, int ntraj, int nb, int nslots,
+                                        double dv, double vcell, double* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntraj * nslots) return;
+  const int t = i / nslots, s = i - t * nslots;
+  const int rows = nb * td[t].nk;
+  const double* p = td[t].obs + (size_t)s * td[t].obs_stride;
+  double ek = 0.0, jj = 0.0, ex = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    ek += p[3 * r + 0];
+    jj += p[3 * r + 1];
+    ex += p[3 * r + 2];
+  }
+  ex = 0.5 * ex;
+  const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
+  double* o = out + (size_t)7 * i;
   o[0] = ek + ei + eh + ex;
   o[1] = n;
   o[2] = -(jj * dv / vcell);

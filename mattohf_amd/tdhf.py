@@ -58,6 +58,18 @@ SIGNATURES = {
     "tdhf_gram_schmidt": (_i, [_vp, _vp]),
     "tdhf_set_kernel_timing": (_i, [_vp, _i]),
     "tdhf_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
+    "tdhf_batch_create": (_i, [_PP, _i, _vp, _vp, C.POINTER(_vp)]),
+    "tdhf_batch_destroy": (_i, [_vp]),
+    "tdhf_batch_last_error": (C.c_char_p, [_vp]),
+    "tdhf_batch_upload": (_i, [_vp, _i, _vp]),
+    "tdhf_batch_download": (_i, [_vp, _i, _vp]),
+    "tdhf_batch_propagate": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tdhf_batch_run": (_i, [_PP, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tdhf_batch_set_waves": (_i, [_vp, _i]),
+    "tdhf_batch_get_waves": (_i, [_vp]),
+    "tdhf_batch_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(_i)]),
+    "tdhf_batch_set_kernel_timing": (_i, [_vp, _i]),
+    "tdhf_batch_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
 }
 
 
@@ -315,4 +327,103 @@ class TDHF:
     def kernel_stats(self):
         n, ms = C.c_long(), C.c_double()
         _chk(load().tdhf_kernel_stats(self.h, C.byref(n), C.byref(ms)), self.h)
+        return n.value, ms.value
+
+
+def _bchk(rc, h):
+    if rc != 0:
+        msg = load().tdhf_batch_last_error(h)
+        raise TDHFError(f"libtdhf error {rc}: {msg.decode() if msg else ''}")
+
+
+class Batch:
+    """Independent trajectories of one cell advanced together (tdhf_batch).
+
+    nks[t] is the k-mesh of trajectory t; vions is (ntraj, NG).  Orbitals are
+    passed as a list of (NG, NB, nk_t) arrays."""
+
+    def __init__(self, ng, nb, nbocc, acell, dt, nks, vions):
+        self.ng, self.nb, self.nbocc = int(ng), int(nb), int(nbocc)
+        self.acell, self.dt = float(acell), float(dt)
+        self.nks = np.ascontiguousarray(nks, dtype=np.int32)
+        self.ntraj = len(self.nks)
+        v = np.ascontiguousarray(vions, dtype=np.float64).reshape(self.ntraj, self.ng)
+        self.params = Params(self.ng, 0, self.nb, self.nbocc, self.acell, self.dt)
+        h = C.c_void_p()
+        rc = load().tdhf_batch_create(C.byref(self.params), self.ntraj, _ptr(self.nks), _ptr(v),
+                                      C.byref(h))
+        if rc != 0:
+            _chk(rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().tdhf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def shapes(self):
+        return [(self.ng, self.nb, int(k)) for k in self.nks]
+
+    def upload(self, us):
+        flat = np.concatenate([np.asfortranarray(u, dtype=np.complex128).ravel(order="F")
+                               for u in us])
+        _bchk(load().tdhf_batch_upload(self.h, -1, _ptr(flat)), self.h)
+
+    def download(self):
+        n = sum(int(np.prod(s)) for s in self.shapes())
+        flat = np.empty(n, dtype=np.complex128)
+        _bchk(load().tdhf_batch_download(self.h, -1, _ptr(flat)), self.h)
+        out, o = [], 0
+        for s in self.shapes():
+            m = int(np.prod(s))
+            out.append(flat[o:o + m].reshape(s, order="F"))
+            o += m
+        return out
+
+    def propagate(self, A, Aave, want_obs=True):
+        """A, Aave: (ntraj, nt).  Returns (etot, nele, jtot) each (ntraj, nt) and init3."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        Aave = np.ascontiguousarray(Aave, dtype=np.float64)
+        nt = A.shape[1]
+        if want_obs:
+            e, n, j = (np.empty((self.ntraj, nt)) for _ in range(3))
+            i3 = np.empty((self.ntraj, 3))
+        else:
+            e = n = j = i3 = None
+        _bchk(load().tdhf_batch_propagate(self.h, nt, _ptr(A), _ptr(Aave), _ptr(e), _ptr(n),
+                                          _ptr(j), _ptr(i3)), self.h)
+        return e, n, j, i3
+
+    def set_waves(self, nw):
+        _bchk(load().tdhf_batch_set_waves(self.h, int(nw)), self.h)
+
+    def waves(self):
+        return load().tdhf_batch_get_waves(self.h)
+
+    def autotune(self, A, Aave):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        Aave = np.ascontiguousarray(Aave, dtype=np.float64)
+        best = C.c_int()
+        _bchk(load().tdhf_batch_autotune(self.h, A.shape[1], _ptr(A), _ptr(Aave), C.byref(best)),
+              self.h)
+        return best.value
+
+    def set_kernel_timing(self, on):
+        _bchk(load().tdhf_batch_set_kernel_timing(self.h, 1 if on else 0), self.h)
+
+    def kernel_stats(self):
+        n, ms = C.c_long(), C.c_double()
+        _bchk(load().tdhf_batch_kernel_stats(self.h, C.byref(n), C.byref(ms)), self.h)
         return n.value, ms.value
