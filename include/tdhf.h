@@ -139,6 +139,29 @@ int tdhf_set_kernel_timing(tdhf_ctx* ctx, int enable);
 int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);
 
 /* ------------------------------------------------------------------------
+ * Differentiation.  No counterpart in the reference: a hand-written adjoint
+ * of the RK4 propagation (RT_RK4_ubk, src/RT_calculation.f90:154-173, with
+ * H = ubkwbk_hwbk, src/Functions.f90:187-209) for the objective
+ *     L(A) = Etot(u_N, A[N-1]) - eref        (Etot as ubk_Etot, :74-91)
+ * after nt steps driven by the per-step field A[0..nt-1] (Aave is derived
+ * from A as Define_Afield does, src/RT_calculation.f90:225-230).
+ * tdhf_fwd runs the forward pass and keeps either every RK4 stage state or,
+ * when those exceed mem_budget bytes (<= 0: a quarter of free device
+ * memory), a checkpoint every S = ceil(sqrt(nt)) steps (or ~sqrt(nt/4), the
+ * least-memory spacing, if that does not fit) with segment recomputation in
+ * tdhf_bwd.  Requires NB == NBocc.
+ *   u     : initial orbitals in, final orbitals out
+ *   L     : objective; etot/nele/jtot: nt observables as tdhf_rt_run (NULL ok)
+ * tdhf_bwd returns dL/dA[0..nt-1] and, if dLdu0 != NULL, dL/du0 as a
+ * Euclidean cotangent (dL = Re sum conj(dLdu0) du0), NG*NB*NK complex. */
+int tdhf_fwd(tdhf_ctx* ctx, double* u, int nt, const double* A, double eref, double mem_budget,
+             double* L, double* etot, double* nele, double* jtot);
+int tdhf_bwd(tdhf_ctx* ctx, double* dLdA, double* dLdu0);
+/* How the last tdhf_fwd stored its states: full_tape = 1 if every stage state
+ * is kept, else segment = checkpoint spacing S. */
+int tdhf_adjoint_info(tdhf_ctx* ctx, int* full_tape, int* segment);
+
+/* ------------------------------------------------------------------------
  * Batched trajectories.  No counterpart in the reference (which runs one
  * trajectory per process): many independent RT runs of one cell (NG, NB,
  * NBocc, acell, dt shared; NK, ion potential and field per trajectory)

@@ -8,8 +8,12 @@
 // error.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
+#include <map>
+#include <numeric>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -18,6 +22,7 @@
 #include "../../include/tdhf.h"
 #include "tdhf_kernels.h"
 #include "tdhf_gs.h"
+#include "tdhf_adjoint.h"
 
 using tdhf::zd;
 
@@ -509,6 +514,7 @@ int ctx_fail(tdhf_ctx* c, int code, const char* msg) { return fail(c, code, "%s"
 }  // namespace tdhf
 
 #include "tdhf_gs_impl.h"
+#include "tdhf_adjoint_impl.h"
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -589,6 +595,7 @@ int tdhf_create(const tdhf_params* p, const double* vion, const double* vcg, con
 int tdhf_destroy(tdhf_ctx* c) {
   if (!c) return 0;
   (void)hipStreamSynchronize(c->stream);
+  adj_release(c);
   release(c);
   delete c;
   return 0;

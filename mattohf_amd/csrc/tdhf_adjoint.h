@@ -1,0 +1,306 @@
+// tdhf_adjoint.h -- backward (adjoint) RK4/TDHF kernels (no counterpart in
+// the reference; they differentiate RT_RK4_ubk, src/RT_calculation.f90
+// :154-173, and ubkwbk_hwbk, src/Functions.f90:187-209).
+//
+// Cotangents are Euclidean: dL = Re sum conj(c) du.  For f(y) = -i H_a[y] y
+// and a cotangent kbar of f, gbar = i kbar is the cotangent of g = H y and
+//   J^T gbar = T_a gbar + (vion + vH) gbar + 2 occ (K s) y
+//            + sum_{q,j} y_qj C_kq(gbar_ki conj y_qj + y_ki conj gbar_qj)
+//            + sum_{q,j} C'_kq(y_ki conj y_qj) gbar_qj
+//            + sum_{q,j} D_kq [FFT(y_ki conj gbar_qj)](NG/2) (-1)^x y_qj
+// with s = Re sum conj(gbar) y, K the Hartree operator, C_kq the exchange
+// convolution (table vt_kq) and C'_kq the same with the table at reversed G,
+// vt'_kq(g) = vt_qk(-g), so that conj(C_kq f) = C'_qk(conj f).  Because the
+// grid holds -NG/2 but not +NG/2, C' differs from C only at g = NG/2:
+// C'_kq f = C_kq f + D_kq fhat(NG/2) (-1)^x, D_kq = vt_qk(NG/2) - vt_kq(NG/2).
+// So each pair costs two transforms; the Nyquist terms are O(NG).  The field
+// derivative of f is -i P_a y (P_a = p + k + a), giving
+//   dL/da = Re sum_G (G + k + a) conj(ghat(G)) yhat(G) / NG.
+#pragma once
+#include "tdhf_kernels.h"
+
+namespace tdhf {
+
+enum AdjMode : int { A_EGRAD = 0, A_S1 = 1, A_S2 = 2, A_S3 = 3, A_S4 = 4 };
+
+struct AdjArgs {
+  int nk, nb, nbocc;
+  double bx, dv, occ, dt;
+  double a;                 // field of the stage (A_EGRAD: A of the final state)
+  const double* kx;
+  const double* vion;
+  const double* vcgl;
+  const double* vtl;
+  const zd* tw;
+  const zd* y;              // stage state (all rows)
+  const zd* gb;             // cotangent gbar of H y (all rows); A_EGRAD: unused
+  const zd* ub;             // cotangent of the state after the step (read only)
+  zd* ua;                   // cotangent accumulator of the state before the step
+  zd* gout;                 // next gbar (A_EGRAD: gbar of stage 4 of the last step)
+  zd* cout;                 // A_EGRAD: dE/du (Euclidean cotangent)
+  double* abar;             // per-row field derivative of this stage, or nullptr
+  int mode;
+};
+
+template <int NG>
+__host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
+  return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
+         5 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double);
+}
+
+template <int NG>
+__global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
+  using F = GroupFFT<NG>;
+  constexpr int R = F::R, L = F::L, GPW = 64 / L;
+  constexpr int JNYQ = L / 2;  // register of lane 0 holding G index NG/2
+  extern __shared__ zd smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  const int l = lane % L, gw = lane / L;
+  const int grp = wave * GPW + gw, ngrp = nw * GPW;
+  const int row = blockIdx.x;
+  const int ik = row / a.nb, ib = row - ik * a.nb;
+  const bool egrad = a.mode == A_EGRAD;
+  const zd* gbar = egrad ? a.y : a.gb;
+
+  zd* T = smem + grp * F::TSZ;
+  zd* sY = smem + ngrp * F::TSZ;   // y_ki
+  zd* sG = sY + NG;                // gbar_ki
+  zd* sACC = sG + NG;              // exchange part of J^T gbar
+  zd* sTG = sACC + NG;             // T gbar
+  zd* sYH = sTG + NG;              // FFT(y_ki) in G layout
+  double* sRho = reinterpret_cast<double*>(sYH + NG);
+  double* sS = sRho + NG;
+  double* sVH = sS + NG;
+  double* sKS = sVH + NG;          // K s (Hartree response)
+  double* sRed = sKS + NG;
+
+  const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
+  zd yv = mk(0.0, 0.0), gv = mk(0.0, 0.0);
+  double vionv = 0.0;
+  if (tid < NG) {
+    yv = a.y[o];
+    gv = gbar[o];
+    sY[tid] = yv;
+    sG[tid] = gv;
+    vionv = a.vion[tid];
+  }
+  const double kxv = a.kx[ik];
+  F fft;
+  fft.init(a.tw, l);
+  const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = L*m + l, L even
+  const bool out_occ = ib < a.nbocc;
+
+  zd acc[R];
+  double den[R], sv[R];
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    acc[m] = mk(0.0, 0.0);
+    den[m] = 0.0;
+    sv[m] = 0.0;
+  });
+  __syncthreads();
+
+  // ---- pair loop over all partner rows (q, j) ----
+  const int nitem = a.nk * a.nb;
+  for (int item = grp; item < nitem; item += ngrp) {
+    const int iq = item / a.nb, jb = item - iq * a.nb;
+    const size_t prow = (size_t)NG * (jb + a.nb * iq);
+    const bool p_occ = jb < a.nbocc;
+    zd uq[R], gq[R], v[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      uq[m] = a.y[prow + L * m + l];
+      gq[m] = egrad ? mk(0.0, 0.0) : a.gb[prow + L * m + l];
+      den[m] += p_occ ? zabs2(uq[m]) : 0.0;
+      sv[m] += gq[m].x * uq[m].x + gq[m].y * uq[m].y;
+    });
+    const double* vt = a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + l * R;
+    // D_kq = vt_qk(NG/2) - vt_kq(NG/2): G-layout position L/2 of rows (q-k), (k-q)
+    const double dnyq = a.vtl[(size_t)(iq - ik + a.nk - 1) * NG + JNYQ] -
+                        a.vtl[(size_t)(ik - iq + a.nk - 1) * NG + JNYQ];
+    double vtr[R];
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      vtr[j] = vt[j];
+    });
+    // T1: phi' = C'_kq(y_ki conj y_qj); adds phi' gbar_qj (or, for dE/du,
+    // (phi + phi') y_qj)
+    if (out_occ && (p_occ || !egrad)) {
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = zmulc(sY[L * m + l], uq[m]);
+      });
+      fft.fwd(v, T, l);
+      // rho_hat(NG/2) sits in lane 0 (of the group), register JNYQ
+      const double nx = __shfl(v[JNYQ].x, gw * L, 64);
+      const double ny = __shfl(v[JNYQ].y, gw * L, 64);
+      static_for<0, R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = zscal(v[j], vtr[j]);
+      });
+      fft.inv(v, T, l);
+      const zd fix = mk(sgn * dnyq * nx, sgn * dnyq * ny);
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const zd ph = v[m];                 // C_kq rho
+        const zd php = zadd(ph, fix);       // C'_kq rho
+        if (egrad) {
+          const zd s2 = zadd(ph, php);
+          acc[m] = zadd(acc[m], zmul(s2, uq[m]));
+        } else {
+          acc[m] = zadd(acc[m], zmul(php, gq[m]));
+        }
+      });
+    }
+    // T2: psi = C_kq(gbar_ki conj y_qj + y_ki conj gbar_qj) + D fhat_C(NG/2) (-1)^x
+    if (!egrad) {
+      double cx = 0.0, cy = 0.0;
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const zd fb = p_occ ? zmulc(sG[L * m + l], uq[m]) : mk(0.0, 0.0);
+        const zd fc = out_occ ? zmulc(sY[L * m + l], gq[m]) : mk(0.0, 0.0);
+        cx += fc.x;
+        cy += fc.y;
+        v[m] = zadd(fb, fc);
+      });
+      cx *= sgn;
+      cy *= sgn;
+#pragma unroll
+      for (int off = 1; off < L; off <<= 1) {
+        cx += __shfl_xor(cx, off);
+        cy += __shfl_xor(cy, off);
+      }
+      fft.fwd(v, T, l);
+      static_for<0, R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = zscal(v[j], vtr[j]);
+      });
+      fft.inv(v, T, l);
+      const zd fix = mk(sgn * dnyq * cx, sgn * dnyq * cy);
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        acc[m] = zadd(acc[m], zmul(zadd(v[m], fix), uq[m]));
+      });
+    }
+  }
+
+  // ---- reductions over the groups (fixed order) ----
+  double* slotd = reinterpret_cast<double*>(T);
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    slotd[2 * (L * m + l)] = acc[m].x;
+    slotd[2 * (L * m + l) + 1] = acc[m].y;
+  });
+  __syncthreads();
+  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
+    reinterpret_cast<double*>(sACC)[idx] = s;
+  }
+  __syncthreads();
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    slotd[L * m + l] = den[m];
+    slotd[NG + L * m + l] = sv[m];
+  });
+  __syncthreads();
+  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
+    if (idx < NG)
+      sRho[idx] = a.occ * s;
+    else
+      sS[idx - NG] = s;
+  }
+  __syncthreads();
+
+  // ---- wave 0: Hartree(rho) | T gbar | Hartree(s) | FFT(y_ki) ----
+  double vcg[R];
+  static_for<0, R>([&](auto jc) {
+    constexpr int j = decltype(jc)::value;
+    vcg[j] = a.vcgl[l * R + j];
+  });
+  if (wave == 0) {
+    zd v[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int x = L * m + l;
+      v[m] = gw == 0 ? mk(sRho[x], 0.0)
+           : gw == 1 ? sG[x]
+           : gw == 2 ? mk(sS[x], 0.0)
+           : gw == 3 ? sY[x]
+                     : mk(0.0, 0.0);
+    });
+    fft.fwd(v, T, l);
+    if (gw == 3) {
+      static_for<0, R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        sYH[l * R + j] = v[j];
+      });
+    }
+    wave_sync();
+    double pa = 0.0;
+    const double kA = kxv + a.a, kA2 = kA * kA;
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const double G = gvalue(F::gindex(l, j), NG, a.bx);
+      if (gw == 0 || gw == 2) {
+        v[j] = zscal(v[j], vcg[j]);
+      } else if (gw == 1) {
+        const zd yh = sYH[l * R + j];
+        pa += (G + kA) * (v[j].x * yh.x + v[j].y * yh.y);
+        v[j] = zscal(v[j], 0.5 * (G * G + 2.0 * G * kA + kA2) / (double)NG);
+      }
+    });
+    fft.inv(v, T, l);
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) pa += __shfl_xor(pa, off);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int x = L * m + l;
+      if (gw == 0) sVH[x] = v[m].x;
+      if (gw == 1) sTG[x] = v[m];
+      if (gw == 2) sKS[x] = v[m].x;
+    });
+    if (gw == 1 && l == 0) sRed[0] = pa / (double)NG;
+  }
+  __syncthreads();
+
+  // ---- epilogue ----
+  if (tid < NG) {
+    const double occr = out_occ ? a.occ : 0.0;
+    if (egrad) {
+      // dE/du = occ dv [2 (T y + (vion + vH) y) + (V_F + V'_F) y]
+      const zd h = zadd(sTG[tid], zscal(yv, vionv + sVH[tid]));
+      const zd c = zscal(zadd(zscal(h, 2.0), sACC[tid]), occr * a.dv);
+      a.cout[o] = c;
+      const double w = a.dt / 6.0;  // gbar_4 = i (dt/6) c
+      a.gout[o] = mk(-w * c.y, w * c.x);
+    } else {
+      zd vj = zadd(sTG[tid], zscal(gv, vionv + sVH[tid]));
+      vj = zadd(vj, zscal(yv, 2.0 * a.occ * (ib < a.nbocc ? 1.0 : 0.0) * sKS[tid]));
+      vj = zadd(vj, sACC[tid]);
+      const zd ubv = a.ub[o];
+      zd uav, kb;
+      if (a.mode == A_S4) {
+        uav = zadd(ubv, vj);
+        kb = zadd(zscal(ubv, a.dt / 3.0), zscal(vj, a.dt));
+      } else if (a.mode == A_S3) {
+        uav = zadd(a.ua[o], vj);
+        kb = zadd(zscal(ubv, a.dt / 3.0), zscal(vj, 0.5 * a.dt));
+      } else if (a.mode == A_S2) {
+        uav = zadd(a.ua[o], vj);
+        kb = zadd(zscal(ubv, a.dt / 6.0), zscal(vj, 0.5 * a.dt));
+      } else {  // A_S1: cotangent of the state before the step; next step's gbar_4
+        uav = zadd(a.ua[o], vj);
+        kb = zscal(uav, a.dt / 6.0);
+      }
+      a.ua[o] = uav;
+      a.gout[o] = mk(-kb.y, kb.x);  // i * kbar
+    }
+  }
+  if (!egrad && a.abar && tid == 0) a.abar[row] = sRed[0];
+}
+
+}  // namespace tdhf

@@ -75,6 +75,9 @@ struct ApplyArgs {
   int mode;
   const TrajDesc* td;       // batched launch: trajectory table, or nullptr
   const int2* rowmap;       // batched launch: (trajectory, row) per workgroup
+  zd* tape;                 // RK4 stage states y1..y4 per step (adjoint), or nullptr
+  int tape_base;            // first step held by the tape
+  size_t nstate;            // NG*NB*NK (tape stride of one state)
 };
 
 __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
@@ -164,6 +167,18 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       Akin = At;
       Aobs = At;
       slot = step + 1;
+    }
+  }
+  // taped forward: the stage states live in the tape slots of this step
+  zd* tape0 = nullptr;
+  if (a.tape && a.step_base) {
+    zd* tp = a.tape + (size_t)(step - a.tape_base) * 4 * a.nstate;
+    switch (a.mode) {
+      case M_S1: tape0 = tp; unext = tp + a.nstate; break;
+      case M_S2: u = w = tp + a.nstate; unext = tp + 2 * a.nstate; break;
+      case M_S3: u = w = tp + 2 * a.nstate; unext = tp + 3 * a.nstate; break;
+      case M_S4: u = w = tp + 3 * a.nstate; break;
+      default: break;
     }
   }
 
@@ -337,6 +352,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       const zd kk = mk(h.y, -h.x);  // -i * H w
       const double hdt = 0.5 * a.dt;
       if (a.mode == M_S1) {
+        if (tape0) tape0[o] = u0v;
         ksum[o] = kk;
         unext[o] = mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y);
       } else if (a.mode == M_S2 || a.mode == M_S3) {
@@ -442,6 +458,10 @@ __global__ void obs_reduce_batch_kernel(const TrajDesc* td, int ntraj, int nb, i
 
 __global__ void step_advance_kernel(int* step_base, int by) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step_base += by;
+}
+
+__global__ void step_set_kernel(int* step_base, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step_base = value;
 }
 
 // k_vCk (src/Functions.f90:33-52) for a list of arguments: one workgroup per

@@ -58,6 +58,9 @@ SIGNATURES = {
     "tdhf_gram_schmidt": (_i, [_vp, _vp]),
     "tdhf_set_kernel_timing": (_i, [_vp, _i]),
     "tdhf_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
+    "tdhf_fwd": (_i, [_vp, _vp, _i, _vp, _d, _d, C.POINTER(_d), _vp, _vp, _vp]),
+    "tdhf_bwd": (_i, [_vp, _vp, _vp]),
+    "tdhf_adjoint_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "tdhf_batch_create": (_i, [_PP, _i, _vp, _vp, C.POINTER(_vp)]),
     "tdhf_batch_destroy": (_i, [_vp]),
     "tdhf_batch_last_error": (C.c_char_p, [_vp]),
@@ -321,6 +324,35 @@ class TDHF:
         _chk(load().tdhf_gram_schmidt(self.h, _ptr(u)), self.h)
         return u
 
+    def fwd(self, ubk, A, eref=0.0, mem_budget=0.0, want_obs=True):
+        """Forward pass of the adjoint pair: L = Etot(u_N, A[N-1]) - eref.
+        Returns (L, u_final, etot, nele, jtot)."""
+        u = self._u(ubk, copy=True)
+        A = _f64(A)
+        nt = A.size
+        e, n, j = (np.empty(nt), np.empty(nt), np.empty(nt)) if want_obs else (None,) * 3
+        L = C.c_double()
+        _chk(load().tdhf_fwd(self.h, _ptr(u), nt, _ptr(A), float(eref), float(mem_budget),
+                             C.byref(L), _ptr(e), _ptr(n), _ptr(j)), self.h)
+        self._fwd_nt = nt
+        return L.value, u, e, n, j
+
+    def bwd(self, want_u0=True):
+        """Backward pass after fwd: (dL/dA[0..N-1], dL/du0 as a Euclidean cotangent
+        or None)."""
+        nt = getattr(self, "_fwd_nt", None)
+        if not nt:
+            raise TDHFError("bwd needs a preceding fwd")
+        g = np.empty(nt)
+        gu = np.empty(self.shape, dtype=np.complex128, order="F") if want_u0 else None
+        _chk(load().tdhf_bwd(self.h, _ptr(g), _ptr(gu)), self.h)
+        return g, gu
+
+    def adjoint_info(self):
+        f, s = C.c_int(), C.c_int()
+        _chk(load().tdhf_adjoint_info(self.h, C.byref(f), C.byref(s)), self.h)
+        return bool(f.value), s.value
+
     def set_kernel_timing(self, on):
         _chk(load().tdhf_set_kernel_timing(self.h, 1 if on else 0), self.h)
 
@@ -427,3 +459,22 @@ class Batch:
         n, ms = C.c_long(), C.c_double()
         _bchk(load().tdhf_batch_kernel_stats(self.h, C.byref(n), C.byref(ms)), self.h)
         return n.value, ms.value
+
+
+# ---------------------------------------------------------------------------
+# pulse parameterisation by piecewise-linear knots (the adjoint workload)
+# ---------------------------------------------------------------------------
+def knot_matrix(nt, dt, T, nknots):
+    """P (nt x nknots): A[it] = sum_m P[it,m] c_m is the piecewise-linear
+    interpolation of knots c_m at tau_m = m T/(nknots-1), times the sin^2
+    envelope sin^2(pi t/T) (zero for t >= T), at t = dt*it (the time of A[it]
+    in Define_Afield)."""
+    t = dt * np.arange(nt)
+    tau = np.linspace(0.0, T, nknots)
+    P = np.zeros((nt, nknots))
+    for m in range(nknots):
+        e = np.zeros(nknots)
+        e[m] = 1.0
+        P[:, m] = np.interp(t, tau, e, right=0.0)
+    env = np.where(t < T, np.sin(np.pi * t / T) ** 2, 0.0)
+    return P * env[:, None]

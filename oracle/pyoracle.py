@@ -70,6 +70,8 @@ def lib():
             "orc_ks_krylov": (None, [vp, vp, i]),
             "orc_subspace_diag": (None, [vp, vp]),
             "orc_gs_run": (None, [vp, vp, i, i, vp, vp, vp]),
+            "orc_adjoint_energy": (d, [vp, vp, d, i, vp, d, vp, vp]),
+            "orc_objective_energy": (d, [vp, vp, d, i, vp, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -245,6 +247,21 @@ class Oracle:
         u = self._u(u).copy(order="F")
         lib().orc_subspace_diag(self.h, _p(u))
         return u
+
+    def objective_energy(self, u0, dt, A, eref=0.0):
+        """L = Etot(u_N, A[N-1]) - eref after len(A) RK4 steps (Aave from A)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        return lib().orc_objective_energy(self.h, _p(self._u(u0)), float(dt), A.size, _p(A),
+                                          float(eref))
+
+    def adjoint_energy(self, u0, dt, A, eref=0.0):
+        """(L, dL/dA[0..N-1], dL/du0 as a Euclidean cotangent)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        g = np.empty(A.size)
+        gu = np.empty(self.sys.shape, dtype=np.complex128, order="F")
+        L = lib().orc_adjoint_energy(self.h, _p(self._u(u0)), float(dt), A.size, _p(A),
+                                     float(eref), _p(g), _p(gu))
+        return L, g, gu
 
     def GS_calculation(self, u, nscf, ncg):
         """Returns (u, etot_hist[0..nscf], epsbk(NB,NK), epserrorbk(NB,NK))."""

@@ -789,3 +789,223 @@ void orc_gs_run(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_h
   }
   if (epsbk || epserr) orc_epsbk(c, ubk, epsbk, epserr);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint (no counterpart in the reference).  Gradient of                  */
+/*   L(A) = Etot(u_N, A[N-1]) - eref                                        */
+/* with respect to the per-step field A[0..N-1] of the RK4 propagation      */
+/* (RT_RK4_ubk, src/RT_calculation.f90:154-173; Aave from A as in           */
+/* Define_Afield :225-230).  Written directly from the chain rule, with the */
+/* exchange VJP as three literal pair transforms (no Nyquist shortcut):     */
+/* cotangents are Euclidean, dL = Re sum conj(c) du.                        */
+/* ------------------------------------------------------------------------ */
+
+/* C_kq f = IFFT[-vCGkk(:,kq)/NG * FFT[f/NK]]; rev != 0 uses the table at
+ * reversed G, vt'_kq(g) = vt_qk((NG-g) mod NG), so that conj(C_kq f) =
+ * C'_qk(conj f) holds exactly (the G grid has -NG/2 but not +NG/2). */
+static void conv_kq(const orc_ctx* c, int ik, int iq, int rev, const zc* f, zc* out, zc* t1,
+                    zc* t2) {
+  const int NG = c->s.ng, NK = c->s.nk;
+  for (int g = 0; g < NG; ++g) t1[g] = f[g] / (double)NK;
+  fft_z(NG, -1, t1, t2);
+  for (int g = 0; g < NG; ++g) {
+    double v;
+    if (!rev) v = c->vcgkk[(size_t)NG * (ik * NK + iq) + g];
+    else v = c->vcgkk[(size_t)NG * (iq * NK + ik) + (NG - g) % NG];
+    t2[g] = -v * t2[g] / (double)NG;
+  }
+  fft_z(NG, +1, t2, out);
+}
+
+/* P_a w = IFFT[(G+k+a) FFT[w]]/NG for one orbital at k-point ik */
+static void pvel(const orc_ctx* c, int ik, double a, const zc* w, zc* out, zc* t1) {
+  const int NG = c->s.ng;
+  fft_z(NG, -1, w, t1);
+  for (int g = 0; g < NG; ++g) t1[g] = (c->gx[g] + c->kx[ik] + a) * t1[g] / (double)NG;
+  fft_z(NG, +1, t1, out);
+}
+
+/* VJP of f(y) = -i H_a[y] y, given cotangent kb of f: ubar += J^T kb,
+ * returns abar = d<kb,f>/da. */
+static double vjp_f(const orc_ctx* c, const zc* y, const zc* kb, double a, zc* ubar) {
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk, NBocc = c->s.nbocc;
+  const size_t n = (size_t)NG * NB * NK;
+  zc* gb = (zc*)malloc(sizeof(zc) * n);
+  for (size_t i = 0; i < n; ++i) gb[i] = I * kb[i]; /* cotangent of g = H y */
+  zc* tmp = (zc*)malloc(sizeof(zc) * n);
+  /* linear part: T_a gb + vion gb + vH gb */
+  orc_tpsi(c, (double*)gb, a, (double*)tmp);
+  double *dns = malloc(sizeof(double) * NG), *vh = malloc(sizeof(double) * NG);
+  orc_dns(c, (const double*)y, dns);
+  orc_vh(c, dns, vh);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib)
+      for (int ig = 0; ig < NG; ++ig) {
+        size_t o = ig + NG * (ib + NB * (size_t)ik);
+        ubar[o] += tmp[o] + (c->vion[ig] + vh[ig]) * gb[o];
+      }
+  /* Hartree response: s(x) = Re sum conj(gb) y, ubar += 2 occ (K s) y */
+  double *s = malloc(sizeof(double) * NG), *ks = malloc(sizeof(double) * NG);
+  for (int ig = 0; ig < NG; ++ig) {
+    double acc = 0;
+    for (int ik = 0; ik < NK; ++ik)
+      for (int ib = 0; ib < NB; ++ib) {
+        size_t o = ig + NG * (ib + NB * (size_t)ik);
+        acc += creal(conj(gb[o]) * y[o]);
+      }
+    s[ig] = acc;
+  }
+  orc_vh(c, s, ks);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib)
+      for (int ig = 0; ig < NG; ++ig) {
+        size_t o = ig + NG * (ib + NB * (size_t)ik);
+        ubar[o] += 2.0 * c->occ[ib + NB * ik] * ks[ig] * y[o];
+      }
+  /* exchange: three transforms per pair */
+  zc *f = malloc(sizeof(zc) * NG), *r = malloc(sizeof(zc) * NG), *t1 = malloc(sizeof(zc) * NG),
+     *t2 = malloc(sizeof(zc) * NG);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      zc* ub = ubar + NG * (ib + NB * (size_t)ik);
+      const zc* yki = y + NG * (ib + NB * (size_t)ik);
+      const zc* gki = gb + NG * (ib + NB * (size_t)ik);
+      for (int iq = 0; iq < NK; ++iq)
+        for (int jb = 0; jb < NB; ++jb) {
+          const zc* yqj = y + NG * (jb + NB * (size_t)iq);
+          const zc* gqj = gb + NG * (jb + NB * (size_t)iq);
+          if (jb < NBocc) { /* from the dependence of V_F y_ki on w = y_ki */
+            for (int g = 0; g < NG; ++g) f[g] = gki[g] * conj(yqj[g]);
+            conv_kq(c, ik, iq, 0, f, r, t1, t2);
+            for (int g = 0; g < NG; ++g) ub[g] += r[g] * yqj[g];
+          }
+          if (ib < NBocc) { /* from the partner dependence of V_F y_qj on y_ki */
+            for (int g = 0; g < NG; ++g) f[g] = yki[g] * conj(yqj[g]);
+            conv_kq(c, ik, iq, 1, f, r, t1, t2);
+            for (int g = 0; g < NG; ++g) ub[g] += r[g] * gqj[g];
+            for (int g = 0; g < NG; ++g) f[g] = yki[g] * conj(gqj[g]);
+            conv_kq(c, ik, iq, 1, f, r, t1, t2);
+            for (int g = 0; g < NG; ++g) ub[g] += r[g] * yqj[g];
+          }
+        }
+    }
+  /* field derivative: d f/da = -i P_a y  ->  abar = Re <gb, P_a y> */
+  double abar = 0.0;
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      const zc* yy = y + NG * (ib + NB * (size_t)ik);
+      pvel(c, ik, a, yy, r, t1);
+      for (int g = 0; g < NG; ++g) abar += creal(conj(gb[g + NG * (ib + NB * (size_t)ik)]) * r[g]);
+    }
+  free(f); free(r); free(t1); free(t2); free(s); free(ks); free(dns); free(vh); free(gb); free(tmp);
+  return abar;
+}
+
+/* cotangent of Etot(u, A) with respect to u, and dEtot/dA = -J vcell */
+static double energy_grad(const orc_ctx* c, const zc* u, double A, zc* cu) {
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk, NBocc = c->s.nbocc;
+  const size_t n = (size_t)NG * NB * NK;
+  zc* h = (zc*)malloc(sizeof(zc) * n);
+  orc_tpsi(c, (const double*)u, A, (double*)h);
+  double *dns = malloc(sizeof(double) * NG), *vh = malloc(sizeof(double) * NG);
+  orc_dns(c, (const double*)u, dns);
+  orc_vh(c, dns, vh);
+  zc *f = malloc(sizeof(zc) * NG), *r = malloc(sizeof(zc) * NG), *t1 = malloc(sizeof(zc) * NG),
+     *t2 = malloc(sizeof(zc) * NG);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      const double occ = c->occ[ib + NB * ik];
+      zc* o = cu + NG * (ib + NB * (size_t)ik);
+      const zc* uk = u + NG * (ib + NB * (size_t)ik);
+      const zc* hk = h + NG * (ib + NB * (size_t)ik);
+      for (int g = 0; g < NG; ++g) o[g] = 2.0 * occ * c->dvcell * (hk[g] + (c->vion[g] + vh[g]) * uk[g]);
+      if (ib >= NBocc) continue;
+      /* Ex = 1/2 sum occ <rho, C rho> dv:  occ dv (V_F u + V'_F u) */
+      for (int iq = 0; iq < NK; ++iq)
+        for (int jb = 0; jb < NBocc; ++jb) {
+          const zc* uq = u + NG * (jb + NB * (size_t)iq);
+          for (int g = 0; g < NG; ++g) f[g] = uk[g] * conj(uq[g]);
+          conv_kq(c, ik, iq, 0, f, r, t1, t2);
+          for (int g = 0; g < NG; ++g) o[g] += occ * c->dvcell * r[g] * uq[g];
+          conv_kq(c, ik, iq, 1, f, r, t1, t2);
+          for (int g = 0; g < NG; ++g) o[g] += occ * c->dvcell * r[g] * uq[g];
+        }
+    }
+  free(f); free(r); free(t1); free(t2); free(h); free(dns); free(vh);
+  return -orc_je(c, (const double*)u, A) * c->vcell;
+}
+
+double orc_adjoint_energy(const orc_ctx* c, const double* u0_, double dt, int nt, const double* A,
+                          double eref, double* dLdA, double* dLdu0) {
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  zc* tape = (zc*)malloc(sizeof(zc) * n * 4 * (size_t)nt);
+  zc* u = (zc*)malloc(sizeof(zc) * n);
+  zc *h = malloc(sizeof(zc) * n), *kk = malloc(sizeof(zc) * n), *ks = malloc(sizeof(zc) * n);
+  double* Aave = (double*)malloc(sizeof(double) * nt);
+  for (int it = 0; it < nt; ++it) Aave[it] = 0.5 * (A[it] + (it > 0 ? A[it - 1] : 0.0));
+  memcpy(u, u0_, sizeof(zc) * n);
+  const double hdt = 0.5 * dt;
+  for (int it = 0; it < nt; ++it) { /* forward, storing y1..y4 */
+    const double a[4] = {2.0 * Aave[it] - A[it], Aave[it], Aave[it], A[it]};
+    const double cc[4] = {hdt, hdt, dt, 0.0};
+    zc* y = tape + n * 4 * (size_t)it;
+    memcpy(y, u, sizeof(zc) * n);
+    for (int s = 0; s < 4; ++s) {
+      orc_hpsi(c, (double*)(y + n * s), (double*)(y + n * s), a[s], (double*)h);
+      for (size_t i = 0; i < n; ++i) kk[i] = -I * h[i];
+      const double w = (s == 0 || s == 3) ? 1.0 : 2.0;
+      for (size_t i = 0; i < n; ++i) ks[i] = (s == 0 ? 0.0 : ks[i]) + w * kk[i];
+      if (s < 3)
+        for (size_t i = 0; i < n; ++i) y[n * (s + 1) + i] = u[i] + cc[s] * kk[i];
+    }
+    for (size_t i = 0; i < n; ++i) u[i] = u[i] + ks[i] * dt / 6.0;
+  }
+  const double L = orc_etot(c, (double*)u, A[nt - 1]) - eref;
+  for (int it = 0; it < nt; ++it) dLdA[it] = 0.0;
+  zc* ub = (zc*)malloc(sizeof(zc) * n);
+  zc* un = (zc*)malloc(sizeof(zc) * n);
+  zc* kb = (zc*)malloc(sizeof(zc) * n);
+  zc* vj = (zc*)malloc(sizeof(zc) * n);
+  dLdA[nt - 1] += energy_grad(c, u, A[nt - 1], ub);
+  for (int it = nt - 1; it >= 0; --it) {
+    const double a[4] = {2.0 * Aave[it] - A[it], Aave[it], Aave[it], A[it]};
+    const double bw[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+    const double cc[4] = {hdt, hdt, dt, 0.0}; /* y_{s+1} = u + cc[s] k_s */
+    zc* y = tape + n * 4 * (size_t)it;
+    double abar[4];
+    memcpy(un, ub, sizeof(zc) * n);
+    /* kbar_s = bw[s] ub + cc[s] * ybar_{s+1} */
+    zc* ybar_next = NULL;
+    for (int s = 3; s >= 0; --s) {
+      for (size_t i = 0; i < n; ++i) kb[i] = bw[s] * ub[i] + (ybar_next ? cc[s] * ybar_next[i] : 0.0);
+      memset(vj, 0, sizeof(zc) * n);
+      abar[s] = vjp_f(c, y + n * s, kb, a[s], vj);
+      for (size_t i = 0; i < n; ++i) un[i] += vj[i];
+      if (!ybar_next) ybar_next = (zc*)malloc(sizeof(zc) * n);
+      memcpy(ybar_next, vj, sizeof(zc) * n);
+    }
+    free(ybar_next);
+    /* stage fields: a1 = A[it-1] (=2 Aave - A), a2 = a3 = Aave, a4 = A[it] */
+    if (it > 0) dLdA[it - 1] += abar[0] + 0.5 * (abar[1] + abar[2]);
+    dLdA[it] += abar[3] + 0.5 * (abar[1] + abar[2]);
+    memcpy(ub, un, sizeof(zc) * n);
+  }
+  if (dLdu0) memcpy(dLdu0, ub, sizeof(zc) * n);
+  free(ub); free(un); free(kb); free(vj); free(tape); free(u); free(h); free(kk); free(ks); free(Aave);
+  return L;
+}
+
+/* The forward objective alone (for finite differences). */
+double orc_objective_energy(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
+                            double eref) {
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  double* u = (double*)malloc(sizeof(zc) * n);
+  memcpy(u, u0, sizeof(zc) * n);
+  for (int it = 0; it < nt; ++it) {
+    const double aave = 0.5 * (A[it] + (it > 0 ? A[it - 1] : 0.0));
+    orc_rk4_step(c, u, dt, aave, A[it]);
+  }
+  const double L = orc_etot(c, u, A[nt - 1]) - eref;
+  free(u);
+  return L;
+}

@@ -71,6 +71,11 @@ void   orc_subspace_diag(const orc_ctx* c, double* ubk);
 void   orc_gs_run(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_hist,
                   double* epsbk, double* epserr);
 
+double orc_adjoint_energy(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
+                          double eref, double* dLdA, double* dLdu0);
+double orc_objective_energy(const orc_ctx* c, const double* u0, double dt, int nt,
+                            const double* A, double eref);
+
 #ifdef __cplusplus
 }
 #endif
