@@ -174,6 +174,97 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
     }
 
 
+ADJ_CFG = dict(acell=8.0, ng=32, nk=32, nb=2, nbocc=2, z=4.0, alpha=1.0, dt=0.025, nt=4000,
+               nknots=200, knot_sigma=0.05, fd_h=1e-5, fd_dirs=8, cpu_steps=150)
+
+
+def adjoint_bench(seed=4, cpu_check=True):
+    """Adjoint workload: the CPU-runnable cell at NK=32, 4000 RK4 steps,
+    A(t) = sin^2(pi t/T) * piecewise-linear(200 knots ~ N(0, 0.05^2)),
+    objective L = E(T) - E_GS.  Reports the cost of one gradient (forward
+    with tape + backward) relative to a plain forward run (the same steps
+    with E/N/J every step), central finite differences (h=1e-5) of L along 8
+    random knot directions, and, on a bounded 150-step prefix, the CPU
+    oracle's adjoint gradient."""
+    from mattohf_amd import tdhf as T
+
+    c = ADJ_CFG
+    ng, nk, nb, a, dt, nt = c["ng"], c["nk"], c["nb"], c["acell"], c["dt"], c["nt"]
+    vion = T.ion_potential(ng, nk, a, [c["z"]], [c["alpha"]], [0.0])
+    G = T.TDHF(ng, nk, nb, c["nbocc"], a, dt, vion)
+    u = G.Gram_Schmidt(T.init_wf(ng, nb, nk, a))
+    ugs, eh, _, _ = G.GS_calculation(u, 200, 4, tol=1e-13)
+    egs = eh[-1]
+    rng = np.random.default_rng(seed)
+    P = T.knot_matrix(nt, dt, nt * dt, c["nknots"])
+    knots = c["knot_sigma"] * rng.standard_normal(c["nknots"])
+    A = P @ knots
+    Aave = 0.5 * (A + np.concatenate([[0.0], A[:-1]]))
+    # plain forward run (graph-replayed steps, observables every step)
+    G.upload(ugs)
+    G.propagate(A[:50], Aave[:50])  # warm-up
+    G.upload(ugs)
+    t0 = time.perf_counter()
+    G.propagate(A, Aave)
+    t_plain = time.perf_counter() - t0
+    # gradient: forward with tape + backward
+    t0 = time.perf_counter()
+    L, _, _, _, _ = G.fwd(ugs, A, eref=egs)
+    t1 = time.perf_counter()
+    gA, _ = G.bwd(want_u0=False)
+    t2 = time.perf_counter()
+    full, seg = G.adjoint_info()
+    gk = P.T @ gA
+    h = c["fd_h"]
+    rels, rels_norm = [], []
+    for _ in range(c["fd_dirs"]):
+        d = rng.standard_normal(c["nknots"])
+        d /= np.linalg.norm(d)
+        lp = G.fwd(ugs, P @ (knots + h * d), eref=egs, want_obs=False)[0]
+        lm = G.fwd(ugs, P @ (knots - h * d), eref=egs, want_obs=False)[0]
+        fd = (lp - lm) / (2 * h)
+        rels.append(abs(fd - gk @ d) / abs(fd))
+        rels_norm.append(abs(fd - gk @ d) / np.linalg.norm(gk))
+    out = {
+        "workload": "adjoint of E(T)-E_GS: a=8, NG=32, NK=32, NB=2, 4000 RK4 steps dt=0.025, "
+                    "200 piecewise-linear knots x sin^2 envelope",
+        "L": L,
+        "t_forward_s": t_plain,
+        "t_fwd_taped_s": t1 - t0,
+        "t_bwd_s": t2 - t1,
+        "gradient_over_forward": (t2 - t0) / t_plain,
+        "tape": "full" if full else f"sqrt(N) checkpoints every {seg} steps",
+        "fd_h": h,
+        "fd_max_rel_err": max(rels),
+        "fd_max_err_over_grad_norm": max(rels_norm),
+        "fd_roundoff_floor": 1e-16 * abs(egs) / h / np.linalg.norm(gk),
+        "fd_dirs": len(rels),
+    }
+    if cpu_check:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+
+        po.set_threads(min(os.cpu_count() or 1, 16))
+        ns = c["cpu_steps"]
+        S = po.System(ng, nk, nb, c["nbocc"], a)
+        vcg, vq = G.tables()
+        full_t = np.empty((nk * nk, ng))
+        for ikr in range(nk):
+            for ikc in range(nk):
+                full_t[ikr * nk + ikc] = vq[ikr - ikc + nk - 1]
+        O = po.Oracle(S, vion, full_t)
+        t0 = time.perf_counter()
+        Lo, go, _ = O.adjoint_energy(ugs, dt, A[:ns], egs)
+        tcpu = time.perf_counter() - t0
+        G.fwd(ugs, A[:ns], eref=egs, want_obs=False)
+        gg, _ = G.bwd(want_u0=False)
+        out["cpu_gradient_check"] = {
+            "steps": ns, "max_rel_diff": float(np.abs(gg - go).max() / np.abs(go).max()),
+            "cpu_seconds": tcpu}
+    G.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +276,7 @@ def main():
     ap.add_argument("--batch-traj", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain device copy of known size (HBM counter calibration)")
     args = ap.parse_args()
@@ -276,6 +368,8 @@ def main():
             "uniform_nk64": bu,
             "batched_vs_plain_at_nk64": bu["traj_steps_per_s"] / value,
         }
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(cpu_check=not args.no_cpu)
     if not args.no_cpu:
         G2 = setup(cfg)[0]
         out["cpu_baseline"] = cpu_baseline(cfg, vion, G2, u, A, Aave, budget_s=args.cpu_seconds)
