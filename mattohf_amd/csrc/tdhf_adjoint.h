@@ -45,7 +45,8 @@ struct AdjArgs {
 template <int NG>
 __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
-         5 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double);
+         5 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
+         ((size_t)64 * nw + 2 * NG) * sizeof(double);
 }
 
 template <int NG>
@@ -74,6 +75,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
   double* sVH = sS + NG;
   double* sKS = sVH + NG;          // K s (Hartree response)
   double* sRed = sKS + NG;
+  double* sPart = sRed + 16;  // blockDim.x + 2*NG doubles for group_reduce
 
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   zd yv = mk(0.0, 0.0), gv = mk(0.0, 0.0);
@@ -193,11 +195,8 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
     slotd[2 * (L * m + l) + 1] = acc[m].y;
   });
   __syncthreads();
-  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
-    reinterpret_cast<double*>(sACC)[idx] = s;
-  }
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
+               reinterpret_cast<double*>(sACC), 1.0);
   __syncthreads();
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
@@ -205,14 +204,9 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
     slotd[NG + L * m + l] = sv[m];
   });
   __syncthreads();
-  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
-    if (idx < NG)
-      sRho[idx] = a.occ * s;
-    else
-      sS[idx - NG] = s;
-  }
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart, sRho, 1.0);
+  __syncthreads();
+  for (int idx = tid; idx < NG; idx += blockDim.x) sRho[idx] *= a.occ;  // sS follows sRho
   __syncthreads();
 
   // ---- wave 0: Hartree(rho) | T gbar | Hartree(s) | FFT(y_ki) ----

@@ -85,13 +85,48 @@ __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
   return g < NG / 2 ? (double)g * bx : (double)(g - NG) * bx;
 }
 
+// Sums, for each of nv values, the slots of ngrp groups (slot g at base +
+// g*stride doubles) into out[0..nv) times scale, using every thread of the
+// workgroup: each thread adds a strided subset of the groups with four
+// independent accumulators, then nv threads add the partials.  The order of
+// additions is fixed, so the result is the same on every run.  part must hold
+// max(blockDim.x, nv) doubles.  Contains __syncthreads: call from uniform
+// control flow.
+__device__ __forceinline__ void group_reduce(const double* base, int stride, int ngrp, int nv,
+                                             double* part, double* out, double scale) {
+  const int nthr = blockDim.x;
+  int P = nthr / nv;
+  if (P < 1) P = 1;
+  if (P > ngrp) P = ngrp;
+  for (int t = threadIdx.x; t < nv * P; t += nthr) {
+    const int idx = t % nv, p = t / nv;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int g = p;
+    for (; g + 3 * P < ngrp; g += 4 * P) {
+      s0 += base[(size_t)g * stride + idx];
+      s1 += base[(size_t)(g + P) * stride + idx];
+      s2 += base[(size_t)(g + 2 * P) * stride + idx];
+      s3 += base[(size_t)(g + 3 * P) * stride + idx];
+    }
+    for (; g < ngrp; g += P) s0 += base[(size_t)g * stride + idx];
+    part[t] = (s0 + s1) + (s2 + s3);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nv; idx += nthr) {
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) s += part[p * nv + idx];
+    out[idx] = scale * s;
+  }
+}
+
 // LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' tiles
 // (transposes during the pair loop, reduction slots after it), then w, V_F w,
 // T w (complex NG each), rho and vH (NG doubles each) and a small scratch.
 template <int NG>
 __host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
-         3 * NG * sizeof(zd) + 2 * NG * sizeof(double) + 16 * sizeof(double);
+         3 * NG * sizeof(zd) + 2 * NG * sizeof(double) + 16 * sizeof(double) +
+         ((size_t)64 * nw + 2 * NG) * sizeof(double);
 }
 
 template <int NG>
@@ -148,6 +183,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   double* sRho = reinterpret_cast<double*>(sTW + NG);
   double* sVH = sRho + NG;
   double* sRed = sVH + NG;
+  double* sPart = sRed + 16;  // blockDim.x + 2*NG doubles for group_reduce
 
   // field values for this launch
   int step = 0, slot = a.obs_slot;
@@ -270,11 +306,8 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     slotd[2 * (L * m + l) + 1] = acc[m].y;
   });
   __syncthreads();
-  for (int idx = tid; idx < 2 * NG; idx += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
-    reinterpret_cast<double*>(sVF)[idx] = s;
-  }
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
+               reinterpret_cast<double*>(sVF), 1.0);
   __syncthreads();
   // round 2: density
   static_for<0, R>([&](auto mc) {
@@ -282,11 +315,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     slotd[L * m + l] = den[m];
   });
   __syncthreads();
-  for (int idx = tid; idx < NG; idx += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < ngrp; ++g) s += reinterpret_cast<const double*>(smem + g * F::TSZ)[idx];
-    sRho[idx] = occ * s;
-  }
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, NG, sPart, sRho, occ);
   __syncthreads();
 
   // ---- wave 0: Hartree (group 0) and kinetic (group 1) round trips ----
