@@ -43,6 +43,7 @@ struct tdhf_ctx {
   // device tables
   double *d_kx = nullptr, *d_vion = nullptr, *d_vcg = nullptr, *d_vcgl = nullptr;
   double *d_vq = nullptr, *d_vtl = nullptr;
+  double* d_kc = nullptr;  // circulant kernels K_H, K_2, K_1 of the apply kernel
   zd* d_tw = nullptr;
   // orbital buffers (NG*NB*NK complex each)
   zd *d_u0 = nullptr, *d_ksum = nullptr, *d_sa = nullptr, *d_sb = nullptr;
@@ -219,8 +220,8 @@ tdhf::ApplyArgs base_args(const tdhf_ctx* c) {
   a.dt = c->dt;
   a.kx = c->d_kx;
   a.vion = c->d_vion;
-  a.vcgl = c->d_vcgl;
   a.vtl = c->d_vtl;
+  a.kc = c->d_kc;
   a.tw = c->d_tw;
   a.obs_stride = c->obs_stride;
   return a;
@@ -471,6 +472,33 @@ int build_tables(tdhf_ctx* c, const double* vcg, const double* vcgkk) {
   }
   HIPCHK(c, hipGetLastError());
   CHKRET(relayout(c, c->d_vcg, 1, 1.0 / (double)NG, c->d_vcgl));
+  {
+    // circulant kernels of the Hartree and kinetic round trips:
+    // K_H(d) = 1/NG sum_g vCG(g) cos(2 pi g d/NG), K_2(d) = 1/NG sum_g G^2/2
+    // cos(2 pi g d/NG), K_1(d) = 1/NG sum_g G exp(2 pi i g d/NG), G in FFT order
+    // (src/Initialization.f90:97-102); summed in long double.
+    std::vector<double> vh(NG), kc(4 * (size_t)NG);
+    HIPCHK(c, hipMemcpy(vh.data(), c->d_vcg, NG * sizeof(double), hipMemcpyDeviceToHost));
+    const long double tpi = 6.283185307179586476925286766559L;
+    for (int d = 0; d < NG; ++d) {
+      long double kh = 0.0L, k2 = 0.0L, k1r = 0.0L, k1i = 0.0L;
+      for (int g = 0; g < NG; ++g) {
+        const long double th = tpi * (long double)((g * d) % NG) / (long double)NG;
+        const long double cs = cosl(th), sn = sinl(th);
+        const long double G = (long double)gx[g];
+        kh += (long double)vh[g] * cs;
+        k2 += 0.5L * G * G * cs;
+        k1r += G * cs;
+        k1i += G * sn;
+      }
+      kc[d] = (double)(kh / NG);
+      kc[NG + d] = (double)(k2 / NG);
+      kc[2 * NG + 2 * d] = (double)(k1r / NG);
+      kc[2 * NG + 2 * d + 1] = (double)(k1i / NG);
+    }
+    CHKRET(dalloc(c, &c->d_kc, kc.size()));
+    HIPCHK(c, hipMemcpy(c->d_kc, kc.data(), kc.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
   CHKRET(relayout(c, c->d_vq, ND, -1.0 / ((double)NG * (double)NK), c->d_vtl));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -481,7 +509,7 @@ void release(tdhf_ctx* c) {
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   c->ev_pool.clear();
   dfree(c->d_kx); dfree(c->d_vion); dfree(c->d_vcg); dfree(c->d_vcgl);
-  dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw);
+  dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw); dfree(c->d_kc);
   dfree(c->d_u0); dfree(c->d_ksum); dfree(c->d_sa); dfree(c->d_sb);
   dfree(c->d_w); dfree(c->d_out);
   dfree(c->d_obs); dfree(c->d_res); dfree(c->d_A); dfree(c->d_Aave); dfree(c->d_step);

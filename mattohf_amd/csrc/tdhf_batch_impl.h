@@ -128,8 +128,8 @@ int batch_update_desc(tdhf_batch* b) {
     d.occ = 2.0 / (double)b->nk[t];
     d.kx = c->d_kx;
     d.vion = b->d_vion + (size_t)b->p.ng * t;
-    d.vcgl = c->d_vcgl;
     d.vtl = c->d_vtl;
+    d.kc = c->d_kc;
     d.u0 = b->d_state + b->off[t];
     d.ksum = b->d_state + b->total + b->off[t];
     d.sa = b->d_state + 2 * b->total + b->off[t];

@@ -9,13 +9,19 @@
 // entirely in registers plus one LDS transpose per transform.  The same pass
 // accumulates the density sum_{q,j} |u(q,j)|^2 (ubk_dns, src/Functions.f90
 // :55-71), so no separate density kernel is needed.  After a workgroup-wide
-// reduction, wave 0 does the Hartree transform (dns_vH, :147-162) and the
-// kinetic transform of w (ubk_tubk, :212-234) side by side, and the epilogue
-// forms H w = T w + (vH + vion) w + V_F w (ubkwbk_hwbk, :187-209) and applies
-// the mode-specific update (plain H|w>, one RK4 stage of RT_RK4_ubk,
+// reduction, the Hartree round trip (dns_vH, :147-162) and the kinetic round
+// trip of w (ubk_tubk, :212-234) are evaluated as the circulant products they
+// are, vH = K_H * rho and T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w, with
+//     K_H(d) = 1/NG sum_G vCG(G) cos(2 pi g d/NG),
+//     K_2(d) = 1/NG sum_G G^2/2 cos(2 pi g d/NG),
+//     K_1(d) = 1/NG sum_G G exp(2 pi i g d/NG)
+// (G in FFT order, Nyquist at -NG/2 b), by all threads of the workgroup
+// instead of one wave's serial transforms.  The epilogue forms
+// H w = T w + (vH + vion) w + V_F w (ubkwbk_hwbk, :187-209) and applies the
+// mode-specific update (plain H|w>, one RK4 stage of RT_RK4_ubk,
 // src/RT_calculation.f90:154-173, or observables only).  Per-row partial sums
-// of Ekin, J and Ex (src/Functions.f90:94-119,165-183,289-318) are produced
-// from the same transforms.
+// of Ekin, J and Ex (src/Functions.f90:94-119,165-183,289-318) follow from the
+// same products by Parseval: sum_G f(G)|w_G|^2/NG = sum_x conj(w) (K_f * w).
 #pragma once
 #include "tdhf_fft.h"
 
@@ -38,8 +44,8 @@ struct TrajDesc {
   double occ;               // 2/nk
   const double* kx;
   const double* vion;
-  const double* vcgl;
   const double* vtl;
+  const double* kc;
   zd* u0;
   zd* ksum;
   zd* sa;
@@ -62,8 +68,8 @@ struct ApplyArgs {
   int obs_stride;           // doubles per observables slot
   const double* kx;         // NK
   const double* vion;       // NG
-  const double* vcgl;       // NG, G layout, vCG(G)/NG
   const double* vtl;        // (2NK-1)*NG, G layout, -v(G+k-q)/(NG*NK)
+  const double* kc;         // 4*NG: circulant kernels K_H, K_2, K_1 (complex)
   const zd* tw;             // twiddle table of GroupFFT<NG> (2*NG)
   const zd* u;              // (NG,NB,NK) orbitals defining rho and V_F
   const zd* w;              // (NG,nbw,NK) orbitals H acts on
@@ -119,14 +125,21 @@ __device__ __forceinline__ void group_reduce(const double* base, int stride, int
   }
 }
 
+// Scratch doubles after the fixed LDS arrays of apply_kernel: the
+// group_reduce partials, max(threads, 2 NG).
+__host__ __device__ constexpr size_t apply_part_doubles(int nw, int NG) {
+  return (size_t)64 * nw > (size_t)2 * NG ? (size_t)64 * nw : (size_t)2 * NG;
+}
+
 // LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' tiles
-// (transposes during the pair loop, reduction slots after it), then w, V_F w,
-// T w (complex NG each), rho and vH (NG doubles each) and a small scratch.
+// (transposes during the pair loop, reduction slots and circulant-product
+// partials after it), then w, V_F w, K_1 (complex NG each), rho, K_H, K_2 (NG
+// doubles each), a small scratch and the reduction scratch.
 template <int NG>
 __host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
-         3 * NG * sizeof(zd) + 2 * NG * sizeof(double) + 16 * sizeof(double) +
-         ((size_t)64 * nw + 2 * NG) * sizeof(double);
+         3 * NG * sizeof(zd) + 3 * NG * sizeof(double) + 16 * sizeof(double) +
+         apply_part_doubles(nw, NG) * sizeof(double);
 }
 
 template <int NG>
@@ -134,6 +147,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
   static_assert(2 * NG <= 2 * F::TSZ, "reduction slot must fit in a tile");
+  // after the reductions the tile area holds 5 partials per thread and the
+  // observables staging (7 NG doubles); checked at the smallest launch
+  // (one wave, or NG/64 waves)
+  static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
+                    2 * F::TSZ * GPW * (NG > 64 ? NG / 64 : 1),
+                "tile area too small for the epilogue scratch");
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -143,7 +162,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   int row = blockIdx.x;
   int nk = a.nk, obs_stride = a.obs_stride;
   double occ = a.occ;
-  const double *kx = a.kx, *vion = a.vion, *vcgl = a.vcgl, *vtl = a.vtl;
+  const double *kx = a.kx, *vion = a.vion, *vtl = a.vtl, *kc = a.kc;
   const double *Aarr = a.Aarr, *Aavearr = a.Aavearr;
   const zd *u = a.u, *w = a.w;
   zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext;
@@ -158,8 +177,8 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     occ = t.occ;
     kx = t.kx;
     vion = t.vion;
-    vcgl = t.vcgl;
     vtl = t.vtl;
+    kc = t.kc;
     Aarr = t.A;
     Aavearr = t.Aave;
     u0 = t.u0;
@@ -179,11 +198,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   zd* T = smem + grp * F::TSZ;
   zd* sW = smem + ngrp * F::TSZ;
   zd* sVF = sW + NG;
-  zd* sTW = sVF + NG;
-  double* sRho = reinterpret_cast<double*>(sTW + NG);
-  double* sVH = sRho + NG;
-  double* sRed = sVH + NG;
-  double* sPart = sRed + 16;  // blockDim.x + 2*NG doubles for group_reduce
+  zd* sK1 = sVF + NG;
+  double* sRho = reinterpret_cast<double*>(sK1 + NG);
+  double* sKH = sRho + NG;
+  double* sK2 = sKH + NG;
+  double* sRed = sK2 + NG;
+  double* sPart = sRed + 16;  // apply_part_doubles(nw, NG)
 
   // field values for this launch
   int step = 0, slot = a.obs_slot;
@@ -225,6 +245,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   if (tid < NG) {
     wv = w[o];
     sW[tid] = wv;
+    sKH[tid] = kc[tid];
+    sK2[tid] = kc[NG + tid];
+    sK1[tid] = reinterpret_cast<const zd*>(kc + 2 * NG)[tid];
     vionv = vion[tid];
     if (a.mode >= M_S1 && a.mode <= M_S4) {
       u0v = u0[o];
@@ -290,13 +313,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     });
   }
 
-  // Hartree multipliers for the tail transform (wave 0, group 0)
-  double vcg[R];
-  static_for<0, R>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    vcg[j] = (wave == 0 && gw == 0) ? vcgl[l * R + j] : 0.0;
-  });
-
   // ---- reduction over all groups through LDS (fixed order) ----
   // round 1: V_F w; each group parks its partial in its own tile
   double* slotd = reinterpret_cast<double*>(T);
@@ -318,63 +334,62 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, NG, sPart, sRho, occ);
   __syncthreads();
 
-  // ---- wave 0: Hartree (group 0) and kinetic (group 1) round trips ----
-  if (wave == 0) {
-    zd v[R];
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      if (gw == 0)
-        v[m] = mk(sRho[L * m + l], 0.0);
-      else if (gw == 1)
-        v[m] = sW[L * m + l];
-      else
-        v[m] = mk(0.0, 0.0);
-    });
-    fft.fwd(v, T, l);
-    double pEk = 0.0, pJ = 0.0;
-    const double kAx = kxv + Akin, kA2 = kAx * kAx;
-    const double kAo = kxv + Aobs, kAo2 = kAo * kAo;
-    static_for<0, R>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      const int g = F::gindex(l, j);
-      const double G = gvalue(g, NG, a.bx);
-      if (gw == 0) {
-        v[j] = zscal(v[j], vcg[j]);
-      } else {
-        const double p2 = zabs2(v[j]);
-        pEk += 0.5 * (G * G + 2.0 * G * kAo + kAo2) * p2;
-        pJ += (G + kxv + Aobs) * p2;
-        const double mult = 0.5 * (G * G + 2.0 * G * kAx + kA2);
-        v[j] = zscal(v[j], mult / (double)NG);
+  // ---- circulant products over all threads: K_H * rho, K_2 * w, K_1 * w ----
+  // thread (p, x) sums the terms y = p, p + P, ... into partials parked in the
+  // tile area (free after the reductions), then thread x adds the P partials
+  // in order.
+  {
+    const int P = blockDim.x / NG;
+    const int x = tid % NG, p = tid / NG;
+    double* cp = reinterpret_cast<double*>(smem);
+    if (p < P) {
+      double vh = 0.0;
+      zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
+      for (int y = p; y < NG; y += P) {
+        const int d = (x - y) & (NG - 1);
+        const double r = sRho[y];
+        const zd wy = sW[y];
+        const double k2 = sK2[d];
+        const zd k1 = sK1[d];
+        vh = fma(sKH[d], r, vh);
+        c2.x = fma(k2, wy.x, c2.x);
+        c2.y = fma(k2, wy.y, c2.y);
+        c1.x = fma(k1.x, wy.x, c1.x);
+        c1.x = fma(-k1.y, wy.y, c1.x);
+        c1.y = fma(k1.x, wy.y, c1.y);
+        c1.y = fma(k1.y, wy.x, c1.y);
       }
-    });
-    fft.inv(v, T, l);
-#pragma unroll
-    for (int off = 1; off < L; off <<= 1) {
-      pEk += __shfl_xor(pEk, off);
-      pJ += __shfl_xor(pJ, off);
-    }
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      if (gw == 0) sVH[L * m + l] = v[m].x;
-      if (gw == 1) sTW[L * m + l] = v[m];
-    });
-    if (gw == 1 && l == 0) {
-      sRed[0] = pEk;
-      sRed[1] = pJ;
+      double* q = cp + 5 * (size_t)tid;
+      q[0] = vh;
+      q[1] = c2.x;
+      q[2] = c2.y;
+      q[3] = c1.x;
+      q[4] = c1.y;
     }
   }
   __syncthreads();
 
   // ---- epilogue: H w and the mode-specific update ----
   const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && obs != nullptr;
-  double pEx = 0.0;
   if (tid < NG) {
+    const double* cp = reinterpret_cast<const double*>(smem);
+    double vh = 0.0;
+    zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
+    const int P = blockDim.x / NG;
+    for (int p = 0; p < P; ++p) {
+      const double* q = cp + 5 * (size_t)(p * NG + tid);
+      vh += q[0];
+      c2.x += q[1];
+      c2.y += q[2];
+      c1.x += q[3];
+      c1.y += q[4];
+    }
+    const double kA = kxv + Akin;
     const zd vf = sVF[tid];
-    zd h = sTW[tid];
-    h = zadd(h, zscal(wv, sVH[tid] + vionv));
+    // T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w
+    zd h = mk(c2.x + kA * c1.x + 0.5 * kA * kA * wv.x, c2.y + kA * c1.y + 0.5 * kA * kA * wv.y);
+    h = zadd(h, zscal(wv, vh + vionv));
     h = zadd(h, vf);
-    pEx = wv.x * vf.x + wv.y * vf.y;
     if (a.mode == M_HPSI) {
       a.out[o] = h;
     } else if (a.mode >= M_S1 && a.mode <= M_S4) {
@@ -392,36 +407,54 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         u0[o] = mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0);
       }
     }
+    if (want_obs) {
+      // per-point terms of Ekin, J, Ex (this row) and N, Eion, EH (row 0)
+      const double kAo = kxv + Aobs;
+      const double w2 = zabs2(wv);
+      const double wc1 = wv.x * c1.x + wv.y * c1.y;  // Re conj(w) (K_1 * w)
+      const double wc2 = wv.x * c2.x + wv.y * c2.y;
+      const double rho = sRho[tid];
+      double* st = reinterpret_cast<double*>(smem) + 5 * (size_t)blockDim.x;
+      st[0 * NG + tid] = wc2 + kAo * wc1 + 0.5 * kAo * kAo * w2;
+      st[1 * NG + tid] = wc1 + kAo * w2;
+      st[2 * NG + tid] = wv.x * vf.x + wv.y * vf.y;
+      st[3 * NG + tid] = rho;
+      st[4 * NG + tid] = vion[tid] * rho;
+      st[5 * NG + tid] = vh * rho;
+    }
   }
   if (want_obs) {
-    // reduce pEx over the NG epilogue threads
-    if (tid < 64) {
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) pEx += __shfl_xor(pEx, off);
-    } else if (tid < 128) {
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) pEx += __shfl_xor(pEx, off);
-      if (tid == 64) sRed[2] = pEx;
+    // six fixed-order sums over x: chunks of 8 points, then the chunks
+    constexpr int NCH = (NG + 7) / 8;
+    const int nv = row == 0 ? 6 : 3;
+    __syncthreads();
+    const double* st = reinterpret_cast<const double*>(smem) + 5 * (size_t)blockDim.x;
+    double* ch = sPart;
+    if (tid < nv * NCH) {
+      const int v = tid / NCH, c = tid - v * NCH;
+      const double* src = st + v * NG + 8 * c;
+      double s = 0.0;
+      for (int i = 0; i < 8 && 8 * c + i < NG; ++i) s += src[i];
+      ch[tid] = s;
     }
     __syncthreads();
     if (tid == 0) {
-      if (NG > 64) pEx += sRed[2];
+      double t[6];
+      for (int v = 0; v < nv; ++v) {
+        double s = 0.0;
+        for (int c = 0; c < NCH; ++c) s += ch[v * NCH + c];
+        t[v] = s;
+      }
       const double occr = ib < a.nbocc ? occ : 0.0;
       double* p = obs + (size_t)slot * obs_stride;
-      p[3 * row + 0] = occr * sRed[0] / (double)NG * a.dv;
-      p[3 * row + 1] = occr * sRed[1] / (double)NG;
-      p[3 * row + 2] = occr * pEx * a.dv;
+      p[3 * row + 0] = occr * t[0] * a.dv;
+      p[3 * row + 1] = occr * t[1];
+      p[3 * row + 2] = occr * t[2] * a.dv;
       if (row == 0) {
         const int rows = a.nbw * nk;
-        double n = 0.0, ei = 0.0, eh = 0.0;
-        for (int x = 0; x < NG; ++x) {
-          n += sRho[x];
-          ei += vion[x] * sRho[x];
-          eh += sVH[x] * sRho[x];
-        }
-        p[3 * rows + 0] = n * a.dv;
-        p[3 * rows + 1] = ei * a.dv;
-        p[3 * rows + 2] = 0.5 * eh * a.dv;
+        p[3 * rows + 0] = t[3] * a.dv;
+        p[3 * rows + 1] = t[4] * a.dv;
+        p[3 * rows + 2] = 0.5 * t[5] * a.dv;
       }
     }
   }
