@@ -40,6 +40,29 @@ __device__ __forceinline__ zd zmulc(zd a, zd b) {
 __device__ __forceinline__ zd zscal(zd a, double s) { return mk(a.x * s, a.y * s); }
 __device__ __forceinline__ double zabs2(zd a) { return a.x * a.x + a.y * a.y; }
 
+// Loads and stores through the global address space.  Buffer pointers that
+// come from tables in device memory are generic, and plain dereferences of
+// them compile to flat instructions; the LDS wait counter also counts flat
+// accesses, so an LDS wait inside a loop would also wait for global loads
+// issued ahead (the prefetch of the next operand).  Every pointer passed here
+// points into device global memory.
+typedef double gd2 __attribute__((ext_vector_type(2)));
+#define TDHF_GLOBAL __attribute__((address_space(1)))
+__device__ __forceinline__ zd gload(const zd* p) {
+  const gd2 v = *(const TDHF_GLOBAL gd2*)(p);
+  return mk(v.x, v.y);
+}
+__device__ __forceinline__ double gload(const double* p) {
+  return *(const TDHF_GLOBAL double*)(p);
+}
+__device__ __forceinline__ void gstore(zd* p, zd v) {
+  gd2 t;
+  t.x = v.x;
+  t.y = v.y;
+  *(TDHF_GLOBAL gd2*)(p) = t;
+}
+__device__ __forceinline__ void gstore(double* p, double v) { *(TDHF_GLOBAL double*)(p) = v; }
+
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
   if constexpr (B < E) {
@@ -127,6 +150,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Workgroup barrier that orders LDS accesses only: global loads issued
+// before it stay in flight (a __syncthreads would wait for them).  For
+// workgroups that exchange data through LDS and never through global memory.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int NG>
 struct Geo;
 template <>
@@ -163,12 +195,12 @@ struct GroupFFT {
   __device__ __forceinline__ void init(const zd* tw, int l) {
     static_for<0, R>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
-      fw[g0] = tw[l * R + g0];
+      fw[g0] = gload(tw + l * R + g0);
     });
     if constexpr (Q > 1) {
       static_for<0, R>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        iw[i] = tw[NG + l * R + i];
+        iw[i] = gload(tw + NG + l * R + i);
       });
     }
   }

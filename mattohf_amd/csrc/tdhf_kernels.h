@@ -96,7 +96,7 @@ __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
 // workgroup: each thread adds a strided subset of the groups with four
 // independent accumulators, then nv threads add the partials.  The order of
 // additions is fixed, so the result is the same on every run.  part must hold
-// max(blockDim.x, nv) doubles.  Contains __syncthreads: call from uniform
+// max(blockDim.x, nv) doubles.  Contains an LDS barrier: call from uniform
 // control flow.
 __device__ __forceinline__ void group_reduce(const double* base, int stride, int ngrp, int nv,
                                              double* part, double* out, double scale) {
@@ -117,11 +117,22 @@ __device__ __forceinline__ void group_reduce(const double* base, int stride, int
     for (; g < ngrp; g += P) s0 += base[(size_t)g * stride + idx];
     part[t] = (s0 + s1) + (s2 + s3);
   }
-  __syncthreads();
+  lds_barrier();
   for (int idx = threadIdx.x; idx < nv; idx += nthr) {
     double s = 0.0;
     for (int p = 0; p < P; ++p) s += part[p * nv + idx];
     out[idx] = scale * s;
+  }
+}
+
+// Sets the issue priority (0..3) of the executing wave; p must be the same in
+// every lane of the wave.
+__device__ __forceinline__ void set_wave_priority(int p) {
+  switch (__builtin_amdgcn_readfirstlane(p)) {
+    case 3: __builtin_amdgcn_s_setprio(3); break;
+    case 2: __builtin_amdgcn_s_setprio(2); break;
+    case 1: __builtin_amdgcn_s_setprio(1); break;
+    default: __builtin_amdgcn_s_setprio(0); break;
   }
 }
 
@@ -210,10 +221,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   double Akin = a.A, Aobs = a.Aobs;
   if (a.step_base) {
     step = *a.step_base + a.jstep;
-    const double At = Aarr[step], Aav = Aavearr[step];
+    const double At = gload(Aarr + step), Aav = gload(Aavearr + step);
     if (a.mode == M_S1) {
       Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
-      Aobs = step > 0 ? Aarr[step - 1] : 0.0;
+      Aobs = step > 0 ? gload(Aarr + step - 1) : 0.0;
       slot = step;
     } else if (a.mode == M_S2 || a.mode == M_S3) {
       Akin = Aav;
@@ -238,23 +249,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
-  // ---- prologue: w of this row into LDS; epilogue operands into registers ----
+  // ---- prologue: w of this row into LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
-  zd wv = mk(0.0, 0.0), u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0);
-  double vionv = 0.0;
-  if (tid < NG) {
-    wv = w[o];
-    sW[tid] = wv;
-    sKH[tid] = kc[tid];
-    sK2[tid] = kc[NG + tid];
-    sK1[tid] = reinterpret_cast<const zd*>(kc + 2 * NG)[tid];
-    vionv = vion[tid];
-    if (a.mode >= M_S1 && a.mode <= M_S4) {
-      u0v = u0[o];
-      if (a.mode != M_S1) ksv = ksum[o];
-    }
-  }
-  const double kxv = kx[ik];
+  if (tid < NG) sW[tid] = gload(w + o);
+  const double kxv = gload(kx + ik);
   F fft;
   fft.init(a.tw, l);
 
@@ -267,36 +265,48 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   });
 
   // ---- exchange pair loop (ubkwbk_vFwbk, src/Functions.f90:247-257) ----
-  // u(q,j) of the next pair is fetched while the current pair is transformed.
+  // u(q,j) of the next pair is fetched while the current pair is transformed;
+  // the multipliers of the current pair are loaded before that prefetch, so
+  // waiting for them leaves the prefetch in flight.
   const int nitem = nk * a.nbocc;
   auto fetch = [&](int item) {
     const int iq = item / a.nbocc, jb = item - iq * a.nbocc;
     const zd* urow = u + (size_t)NG * (jb + a.nb * iq);
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      un[m] = urow[L * m + l];
+      un[m] = gload(urow + L * m + l);
     });
   };
   if (grp < nitem) fetch(grp);
-  __syncthreads();  // sW visible
-  for (int item = grp; item < nitem; item += ngrp) {
+  // The two waves sharing a SIMD would otherwise finish far apart (the older
+  // one is issued first whenever both are ready, and the other then runs the
+  // rest of its items alone).  Each wave lowers its issue priority as it
+  // passes quarters of its items, so a wave that falls behind is issued first
+  // until it catches up.
+  const int wave_items = (nitem - wave * GPW + ngrp - 1) / ngrp;  // of its first group
+  lds_barrier();  // sW visible
+  int it = 0;
+  for (int item = grp; item < nitem; item += ngrp, ++it) {
+    set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
     zd uq[R], v[R];
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       uq[m] = un[m];
     });
-    if (item + ngrp < nitem) fetch(item + ngrp);
     const int iq = item / a.nbocc;
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      v[m] = zmulc(sW[L * m + l], uq[m]);
-      den[m] += zabs2(uq[m]);
-    });
     const double* vt = vtl + (size_t)(ik - iq + nk - 1) * NG + l * R;
     double vtr[R];
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      vtr[j] = vt[j];
+      vtr[j] = gload(vt + j);
+    });
+    // unconditional (the last pass re-reads its own item), so the wait for
+    // vtr above counts exactly the loads issued after it
+    fetch(item + ngrp < nitem ? item + ngrp : item);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[m] = zmulc(sW[L * m + l], uq[m]);
+      den[m] += zabs2(uq[m]);
     });
     fft.fwd(v, T, l);
     static_for<0, R>([&](auto jc) {
@@ -312,6 +322,22 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       acc[m].y = fma(v[m].y, uq[m].x, acc[m].y);
     });
   }
+  set_wave_priority(0);
+
+  // epilogue operands: loaded now, used after the reductions (whose barriers
+  // order LDS only, so these loads stay in flight meanwhile)
+  zd u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0), k1v = mk(0.0, 0.0);
+  double vionv = 0.0, khv = 0.0, k2v = 0.0;
+  if (tid < NG) {
+    khv = gload(kc + tid);
+    k2v = gload(kc + NG + tid);
+    k1v = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
+    vionv = gload(vion + tid);
+    if (a.mode >= M_S1 && a.mode <= M_S4) {
+      u0v = gload(u0 + o);
+      if (a.mode != M_S1) ksv = gload(ksum + o);
+    }
+  }
 
   // ---- reduction over all groups through LDS (fixed order) ----
   // round 1: V_F w; each group parks its partial in its own tile
@@ -321,18 +347,23 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     slotd[2 * (L * m + l)] = acc[m].x;
     slotd[2 * (L * m + l) + 1] = acc[m].y;
   });
-  __syncthreads();
+  lds_barrier();
   group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
                reinterpret_cast<double*>(sVF), 1.0);
-  __syncthreads();
+  lds_barrier();
   // round 2: density
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     slotd[L * m + l] = den[m];
   });
-  __syncthreads();
+  lds_barrier();
   group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, NG, sPart, sRho, occ);
-  __syncthreads();
+  if (tid < NG) {
+    sKH[tid] = khv;
+    sK2[tid] = k2v;
+    sK1[tid] = k1v;
+  }
+  lds_barrier();
 
   // ---- circulant products over all threads: K_H * rho, K_2 * w, K_1 * w ----
   // thread (p, x) sums the terms y = p, p + P, ... into partials parked in the
@@ -367,11 +398,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       q[4] = c1.y;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- epilogue: H w and the mode-specific update ----
   const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && obs != nullptr;
   if (tid < NG) {
+    const zd wv = sW[tid];
     const double* cp = reinterpret_cast<const double*>(smem);
     double vh = 0.0;
     zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
@@ -391,20 +423,21 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     h = zadd(h, zscal(wv, vh + vionv));
     h = zadd(h, vf);
     if (a.mode == M_HPSI) {
-      a.out[o] = h;
+      gstore(a.out + o, h);
     } else if (a.mode >= M_S1 && a.mode <= M_S4) {
       const zd kk = mk(h.y, -h.x);  // -i * H w
       const double hdt = 0.5 * a.dt;
       if (a.mode == M_S1) {
-        if (tape0) tape0[o] = u0v;
-        ksum[o] = kk;
-        unext[o] = mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y);
+        if (tape0) gstore(tape0 + o, u0v);
+        gstore(ksum + o, kk);
+        gstore(unext + o, mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y));
       } else if (a.mode == M_S2 || a.mode == M_S3) {
-        ksum[o] = mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y);
+        gstore(ksum + o, mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y));
         const double c = (a.mode == M_S2) ? hdt : a.dt;
-        unext[o] = mk(u0v.x + c * kk.x, u0v.y + c * kk.y);
+        gstore(unext + o, mk(u0v.x + c * kk.x, u0v.y + c * kk.y));
       } else {
-        u0[o] = mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0);
+        gstore(u0 + o,
+               mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0, u0v.y + (ksv.y + kk.y) * a.dt / 6.0));
       }
     }
     if (want_obs) {
@@ -419,7 +452,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       st[1 * NG + tid] = wc1 + kAo * w2;
       st[2 * NG + tid] = wv.x * vf.x + wv.y * vf.y;
       st[3 * NG + tid] = rho;
-      st[4 * NG + tid] = vion[tid] * rho;
+      st[4 * NG + tid] = vionv * rho;
       st[5 * NG + tid] = vh * rho;
     }
   }
@@ -427,7 +460,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     // six fixed-order sums over x: chunks of 8 points, then the chunks
     constexpr int NCH = (NG + 7) / 8;
     const int nv = row == 0 ? 6 : 3;
-    __syncthreads();
+    lds_barrier();
     const double* st = reinterpret_cast<const double*>(smem) + 5 * (size_t)blockDim.x;
     double* ch = sPart;
     if (tid < nv * NCH) {
@@ -437,7 +470,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       for (int i = 0; i < 8 && 8 * c + i < NG; ++i) s += src[i];
       ch[tid] = s;
     }
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
       double t[6];
       for (int v = 0; v < nv; ++v) {
@@ -447,14 +480,14 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       }
       const double occr = ib < a.nbocc ? occ : 0.0;
       double* p = obs + (size_t)slot * obs_stride;
-      p[3 * row + 0] = occr * t[0] * a.dv;
-      p[3 * row + 1] = occr * t[1];
-      p[3 * row + 2] = occr * t[2] * a.dv;
+      gstore(p + 3 * row + 0, occr * t[0] * a.dv);
+      gstore(p + 3 * row + 1, occr * t[1]);
+      gstore(p + 3 * row + 2, occr * t[2] * a.dv);
       if (row == 0) {
         const int rows = a.nbw * nk;
-        p[3 * rows + 0] = t[3] * a.dv;
-        p[3 * rows + 1] = t[4] * a.dv;
-        p[3 * rows + 2] = 0.5 * t[5] * a.dv;
+        gstore(p + 3 * rows + 0, t[3] * a.dv);
+        gstore(p + 3 * rows + 1, t[4] * a.dv);
+        gstore(p + 3 * rows + 2, 0.5 * t[5] * a.dv);
       }
     }
   }
