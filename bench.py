@@ -34,6 +34,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TDHF traj-RK4-steps/s batched vs plain FP64 (rel dJ<1e-8); adjoint/fwd cost"
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes per apply_kernel<64> launch of the plain workload, from rocprofv3
+# FETCH_SIZE / WRITE_SIZE per dispatch scaled by a plain 1 GiB device copy
+# (FETCH x2.0, WRITE x1.0): profiles/apply_kernel_ng64_nk64_nb4_counters.md
+TRAFFIC_KIB = {"fetch_kib": 1569.9, "fetch_scale": 2.0, "write_kib": 440.3, "write_scale": 1.0}
 
 CFG = dict(acell=10.0, ng=64, nk=64, nb=4, nbocc=4, z=8.0, alpha=1.0, dt=0.01, A0=0.3,
            omega=0.057, T=400.0)
@@ -350,7 +354,9 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": (TRAFFIC_KIB["fetch_kib"] * TRAFFIC_KIB["fetch_scale"]
+                        + TRAFFIC_KIB["write_kib"] * TRAFFIC_KIB["write_scale"]) * 1024.0,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "kernel": "apply_kernel<64> (fused exchange + H + RK4 stage)",
             "avg_launch_us": avg_s * 1e6,
             "flop_per_launch": f_launch,
