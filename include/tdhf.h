@@ -157,6 +157,26 @@ int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);
 int tdhf_fwd(tdhf_ctx* ctx, double* u, int nt, const double* A, double eref, double mem_budget,
              double* L, double* etot, double* nele, double* jtot);
 int tdhf_bwd(tdhf_ctx* ctx, double* dLdA, double* dLdu0);
+
+/* Objectives of the forward/backward pair. */
+#define TDHF_OBJ_ENERGY 0 /* L = Etot(u_N, A[N-1]) - eref                          */
+#define TDHF_OBJ_JSQ 1    /* L = dt * sum_{n=1..N} J_n^2, J_n = ubk_Je(u_n, A[n-1])  */
+
+/* The same pair for ntraj trajectories of the context's cell at once (own
+ * initial orbitals and field each; same NK, tables and ion potential), one
+ * launch per RK4 stage for all of them (adjoint stages likewise), with either
+ * objective.  The memory budget covers the stage states of all trajectories.
+ *   u      : ntraj states concatenated (NG*NB*NK complex each), in: initial,
+ *            out: final
+ *   A      : ntraj*nt, trajectory t at A + nt*t
+ *   eref   : ntraj reference energies (TDHF_OBJ_ENERGY; NULL = 0)
+ *   L      : ntraj objectives; etot/nele/jtot: ntraj*nt (NULL ok)
+ * tdhf_bwd_batch: dLdA ntraj*nt; dLdu0 (NULL ok) ntraj states.
+ * tdhf_fwd / tdhf_bwd are this pair with ntraj = 1 and TDHF_OBJ_ENERGY. */
+int tdhf_fwd_batch(tdhf_ctx* ctx, int ntraj, double* u, int nt, const double* A, int objective,
+                   const double* eref, double mem_budget, double* L, double* etot, double* nele,
+                   double* jtot);
+int tdhf_bwd_batch(tdhf_ctx* ctx, double* dLdA, double* dLdu0);
 /* How the last tdhf_fwd stored its states: full_tape = 1 if every stage state
  * is kept, else segment = checkpoint spacing S. */
 int tdhf_adjoint_info(tdhf_ctx* ctx, int* full_tape, int* segment);

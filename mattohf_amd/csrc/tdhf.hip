@@ -165,11 +165,13 @@ int choose_nw(int ng, int nk, int nbocc) {
 }
 
 template <int NG>
-void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int nw, size_t lds, hipStream_t s) {
-  hipLaunchKernelGGL(tdhf::apply_kernel<NG>, dim3(rows), dim3(64 * nw), lds, s, a);
+void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int ntraj, int nw, size_t lds,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(tdhf::apply_kernel<NG>, dim3(rows, ntraj), dim3(64 * nw), lds, s, a);
 }
 
-int launch_apply(tdhf_ctx* c, const tdhf::ApplyArgs& a, int rows) {
+// rows workgroups per trajectory; ntraj > 1 for a uniform batch (grid y)
+int launch_apply(tdhf_ctx* c, const tdhf::ApplyArgs& a, int rows, int ntraj = 1) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
     if (c->ev_used + 2 > c->ev_pool.size()) {
@@ -184,10 +186,10 @@ int launch_apply(tdhf_ctx* c, const tdhf::ApplyArgs& a, int rows) {
     HIPCHK(c, hipEventRecord(e0, c->stream));
   }
   switch (c->ng) {
-    case 16: launch_apply_t<16>(a, rows, c->nw, c->lds, c->stream); break;
-    case 32: launch_apply_t<32>(a, rows, c->nw, c->lds, c->stream); break;
-    case 64: launch_apply_t<64>(a, rows, c->nw, c->lds, c->stream); break;
-    case 128: launch_apply_t<128>(a, rows, c->nw, c->lds, c->stream); break;
+    case 16: launch_apply_t<16>(a, rows, ntraj, c->nw, c->lds, c->stream); break;
+    case 32: launch_apply_t<32>(a, rows, ntraj, c->nw, c->lds, c->stream); break;
+    case 64: launch_apply_t<64>(a, rows, ntraj, c->nw, c->lds, c->stream); break;
+    case 128: launch_apply_t<128>(a, rows, ntraj, c->nw, c->lds, c->stream); break;
     default: return fail(c, TDHF_EINVAL, "unsupported NG %d", c->ng);
   }
   HIPCHK(c, hipGetLastError());

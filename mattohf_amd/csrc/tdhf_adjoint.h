@@ -26,7 +26,12 @@ enum AdjMode : int { A_EGRAD = 0, A_S1 = 1, A_S2 = 2, A_S3 = 3, A_S4 = 4 };
 struct AdjArgs {
   int nk, nb, nbocc;
   double bx, dv, occ, dt;
-  double a;                 // field of the stage (A_EGRAD: A of the final state)
+  // field of the stage, from the per-step arrays of each trajectory: stage 1
+  // 2 Aave[step] - A[step] (= A(step-1)), stages 2, 3 Aave[step], stage 4 and
+  // A_EGRAD A[step]
+  const double* Aarr;
+  const double* Aavearr;
+  int step;
   const double* kx;
   const double* vion;
   const double* vcgl;
@@ -40,6 +45,11 @@ struct AdjArgs {
   zd* cout;                 // A_EGRAD: dE/du (Euclidean cotangent)
   double* abar;             // per-row field derivative of this stage, or nullptr
   int mode;
+  // uniform batch (grid y = trajectory): offsets of trajectory t are
+  // t*tstride (states, zd), t*fstride (fields), t*abar_tstride (abar)
+  size_t tstride;
+  int fstride;
+  size_t abar_tstride;
 };
 
 template <int NG>
@@ -50,7 +60,26 @@ __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
 }
 
 template <int NG>
-__global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
+__global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
+  // per-trajectory view of the arguments
+  AdjArgs a = a0;
+  const int tr = blockIdx.y;
+  {
+    const size_t so = (size_t)tr * a0.tstride;
+    a.y += so;
+    if (a.gb) a.gb += so;
+    if (a.ub) a.ub += so;
+    if (a.ua) a.ua += so;
+    if (a.gout) a.gout += so;
+    if (a.cout) a.cout += so;
+    if (a.abar) a.abar += (size_t)tr * a0.abar_tstride;
+  }
+  double fa;
+  {
+    const double At = a.Aarr[(size_t)tr * a.fstride + a.step];
+    const double Av = a.Aavearr[(size_t)tr * a.fstride + a.step];
+    fa = a.mode == A_S1 ? 2.0 * Av - At : (a.mode == A_S2 || a.mode == A_S3) ? Av : At;
+  }
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
   constexpr int JNYQ = L / 2;  // register of lane 0 holding G index NG/2
@@ -235,7 +264,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
     }
     wave_sync();
     double pa = 0.0;
-    const double kA = kxv + a.a, kA2 = kA * kA;
+    const double kA = kxv + fa, kA2 = kA * kA;
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       const double G = gvalue(F::gindex(l, j), NG, a.bx);
@@ -295,6 +324,72 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a) {
     }
   }
   if (!egrad && a.abar && tid == 0) a.abar[row] = sRed[0];
+}
+
+// Current-squared objective L = dt sum_n J_n^2 (ubk_Je, src/Functions.f90
+// :289-318): for every row of every trajectory of a uniform batch,
+//   c = 2 dt J[step] (-2 occ dv / vcell) P_A y,   P_A y = K_1 * y + (k + A) y
+// with A = A[step], is the cotangent dL/dy of the state y after step 'step'.
+// init != 0: ua = c and gb = i (dt/6) c (start of the reverse sweep);
+// otherwise both are added (the state cotangent and the stage-4 cotangent of
+// the step before, which the stage-1 adjoint launch has just formed).
+struct JgradArgs {
+  int nb, nbocc;
+  double dt, dv, vcell, occ;
+  const double* kx;
+  const double* kc;         // circulant kernels; K_1 (complex) at kc + 2 NG
+  const double* Aarr;       // per-step A of each trajectory
+  const double* J;          // per-step J of each trajectory
+  int step;
+  const zd* y;              // state after step 'step' (all trajectories)
+  zd* ua;
+  zd* gb;
+  int init;
+  size_t tstride;
+  int fstride;
+};
+
+template <int NG>
+__global__ void __launch_bounds__(NG) jgrad_kernel(JgradArgs a) {
+  __shared__ zd sy[NG];
+  __shared__ zd sk[NG];
+  const int x = threadIdx.x, row = blockIdx.x, tr = blockIdx.y;
+  const int ik = row / a.nb, ib = row - ik * a.nb;
+  const size_t o = (size_t)tr * a.tstride + (size_t)NG * row + x;
+  sy[x] = gload(a.y + o);
+  sk[x] = gload(reinterpret_cast<const zd*>(a.kc + 2 * NG) + x);
+  __syncthreads();
+  const double A = gload(a.Aarr + (size_t)tr * a.fstride + a.step);
+  const double J = gload(a.J + (size_t)tr * a.fstride + a.step);
+  zd p = zscal(sy[x], gload(a.kx + ik) + A);
+  for (int t = 0; t < NG; ++t) p = zadd(p, zmul(sk[(x - t) & (NG - 1)], sy[t]));
+  const double occr = ib < a.nbocc ? a.occ : 0.0;
+  const zd c = zscal(p, 2.0 * a.dt * J * (-2.0 * occr * a.dv / a.vcell));
+  const double w = a.dt / 6.0;
+  const zd g = mk(-w * c.y, w * c.x);
+  if (a.init) {
+    gstore(a.ua + o, c);
+    gstore(a.gb + o, g);
+  } else {
+    gstore(a.ua + o, zadd(gload(a.ua + o), c));
+    gstore(a.gb + o, zadd(gload(a.gb + o), g));
+  }
+}
+
+// out[i] = sum_r in[i*rows + r] (fixed order), for the per-row field
+// derivatives of the adjoint stages.
+__global__ void row_sum_kernel(const double* in, int rows, long n, double* out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = in + (size_t)i * rows;
+  double s0 = 0.0, s1 = 0.0;
+  int r = 0;
+  for (; r + 1 < rows; r += 2) {
+    s0 += p[r];
+    s1 += p[r + 1];
+  }
+  if (r < rows) s0 += p[r];
+  out[i] = s0 + s1;
 }
 
 }  // namespace tdhf

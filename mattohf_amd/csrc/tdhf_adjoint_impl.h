@@ -1,32 +1,45 @@
 // tdhf_adjoint_impl.h -- host side of the forward/backward pair (included by
 // tdhf.hip after the single-trajectory helpers).
 //
-// tdhf_fwd propagates nt RK4 steps and keeps what the backward sweep needs:
-//   - when 4*nt stage states fit the memory budget, every stage state y1..y4
-//     of every step (the forward kernels write them straight into the tape);
+// A forward call propagates ntraj trajectories of the context's cell (same
+// NK, tables and ion potential; own initial orbitals and field each) for nt
+// RK4 steps in one launch per stage (grid y = trajectory) and keeps what the
+// backward sweep needs:
+//   - when 4*nt stage states per trajectory fit the memory budget, every
+//     stage state y1..y4 of every step (the forward kernels write them
+//     straight into the tape);
 //   - otherwise the state at every S-th step, S = ceil(sqrt(nt)) (or the
-//     least-memory spacing ~sqrt(nt/4) if that does not fit), and tdhf_bwd
-//     recomputes each segment once, writing its stage states into a segment
-//     tape.
-// tdhf_bwd starts from dE/du at the final state and runs the reverse RK4
-// stages (adjoint_kernel, two exchange transforms per pair), collecting
-// dL/dA per step; A enters step it as A(it-1) (stage 1), Aave(it) (stages 2,
-// 3) and A(it) (stage 4), with Aave(it) = (A(it) + A(it-1))/2.
+//     least-memory spacing ~sqrt(nt/4) if that does not fit), and the
+//     backward call recomputes each segment once, writing its stage states
+//     into a segment tape.
+// The backward call starts from the cotangent of the final state (dE/du for
+// the energy objective, 2 dt J dJ/du for the current-squared one) and runs
+// the reverse RK4 stages (adjoint_kernel, two exchange transforms per pair),
+// adding 2 dt J_n dJ_n/du at every step boundary for the current-squared
+// objective and collecting dL/dA per step; A enters step it as A(it-1)
+// (stage 1), Aave(it) (stages 2, 3) and A(it) (stage 4), with
+// Aave(it) = (A(it) + A(it-1))/2.
+//
+// Tape layout: [step][stage][trajectory][state], so the stage-s states of one
+// step form one batched state array.
 #pragma once
 
 struct AdjState {
   bool have_fwd = false;
-  int nt = 0;
-  int seg = 0;            // steps per segment (== nt when the whole tape is kept)
+  int nt = 0, ntraj = 1, objective = TDHF_OBJ_ENERGY;
+  int seg = 0;             // steps per segment (== nt when the whole tape is kept)
   bool full = false;
-  std::vector<double> A, Aave;
-  double jN = 0.0;
-  zd* tape = nullptr;     // 4 * seg states
-  zd* ckpt = nullptr;     // ceil(nt/seg) states (segment starts), checkpoint mode
-  zd* buf = nullptr;      // UA | UB | GA | GB
-  double* abar = nullptr; // nt * 4 * rows
-  size_t tape_states = 0, ckpt_states = 0;
-  long abar_cap = 0;
+  std::vector<double> A, Aave;   // ntraj * nt, trajectory-major
+  std::vector<double> jt, nel;   // J and N after every step, ntraj * nt
+  double *dA = nullptr, *dAave = nullptr, *dJ = nullptr;
+  zd* st = nullptr;        // u0 | ksum | sa | sb, ntraj states each
+  zd* tape = nullptr;      // 4 * seg stage blocks of ntraj states
+  zd* ckpt = nullptr;      // ceil(nt/seg) blocks of ntraj states (checkpoint mode)
+  zd* buf = nullptr;       // UA | UB | GA | GB, ntraj states each
+  double* obs = nullptr;   // ntraj * (nt+1) observables slots
+  double* res = nullptr;   // ntraj * (nt+1) * 7 reduced observables
+  double* abar = nullptr;  // ntraj * seg * 4 * rows per-row field derivatives
+  double* stsum = nullptr; // ntraj * nt * 4 stage field derivatives
 };
 
 namespace {
@@ -37,10 +50,17 @@ std::map<const tdhf_ctx*, AdjState>& adj_states() {
 }
 
 void adj_free(AdjState& s) {
+  dfree(s.dA);
+  dfree(s.dAave);
+  dfree(s.dJ);
+  dfree(s.st);
   dfree(s.tape);
   dfree(s.ckpt);
   dfree(s.buf);
+  dfree(s.obs);
+  dfree(s.res);
   dfree(s.abar);
+  dfree(s.stsum);
   s = AdjState();
 }
 
@@ -54,7 +74,7 @@ size_t adj_lds_for(int ng, int nw) {
   return 0;
 }
 
-int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a) {
+int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
   const int rows = c->nb * c->nk;
   int nw = choose_nw(c->ng, c->nk, c->nb);
   const size_t lds = adj_lds_for(c->ng, nw);
@@ -79,18 +99,50 @@ int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a) {
   }
   if (lds > 64 * 1024)
     HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const dim3 grid(rows, ntraj), block(64 * nw);
   switch (c->ng) {
-    case 16: hipLaunchKernelGGL(tdhf::adjoint_kernel<16>, dim3(rows), dim3(64 * nw), lds, c->stream, a); break;
-    case 32: hipLaunchKernelGGL(tdhf::adjoint_kernel<32>, dim3(rows), dim3(64 * nw), lds, c->stream, a); break;
-    case 64: hipLaunchKernelGGL(tdhf::adjoint_kernel<64>, dim3(rows), dim3(64 * nw), lds, c->stream, a); break;
-    case 128: hipLaunchKernelGGL(tdhf::adjoint_kernel<128>, dim3(rows), dim3(64 * nw), lds, c->stream, a); break;
+    case 16: hipLaunchKernelGGL(tdhf::adjoint_kernel<16>, grid, block, lds, c->stream, a); break;
+    case 32: hipLaunchKernelGGL(tdhf::adjoint_kernel<32>, grid, block, lds, c->stream, a); break;
+    case 64: hipLaunchKernelGGL(tdhf::adjoint_kernel<64>, grid, block, lds, c->stream, a); break;
+    case 128: hipLaunchKernelGGL(tdhf::adjoint_kernel<128>, grid, block, lds, c->stream, a); break;
   }
   HIPCHK(c, hipGetLastError());
   if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
   return 0;
 }
 
-tdhf::AdjArgs adj_base(const tdhf_ctx* c) {
+int launch_jgrad(tdhf_ctx* c, const AdjState& s, int step, const zd* y, zd* ua, zd* gb,
+                 int init) {
+  tdhf::JgradArgs j{};
+  j.nb = c->nb;
+  j.nbocc = c->nbocc;
+  j.dt = c->dt;
+  j.dv = c->dv;
+  j.vcell = c->vcell;
+  j.occ = c->occ;
+  j.kx = c->d_kx;
+  j.kc = c->d_kc;
+  j.Aarr = s.dA;
+  j.J = s.dJ;
+  j.step = step;
+  j.y = y;
+  j.ua = ua;
+  j.gb = gb;
+  j.init = init;
+  j.tstride = nstate(c);
+  j.fstride = s.nt;
+  const dim3 grid(c->nb * c->nk, s.ntraj);
+  switch (c->ng) {
+    case 16: hipLaunchKernelGGL(tdhf::jgrad_kernel<16>, grid, dim3(16), 0, c->stream, j); break;
+    case 32: hipLaunchKernelGGL(tdhf::jgrad_kernel<32>, grid, dim3(32), 0, c->stream, j); break;
+    case 64: hipLaunchKernelGGL(tdhf::jgrad_kernel<64>, grid, dim3(64), 0, c->stream, j); break;
+    case 128: hipLaunchKernelGGL(tdhf::jgrad_kernel<128>, grid, dim3(128), 0, c->stream, j); break;
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+tdhf::AdjArgs adj_base(const tdhf_ctx* c, const AdjState& s) {
   tdhf::AdjArgs a{};
   a.nk = c->nk;
   a.nb = c->nb;
@@ -104,204 +156,299 @@ tdhf::AdjArgs adj_base(const tdhf_ctx* c) {
   a.vcgl = c->d_vcgl;
   a.vtl = c->d_vtl;
   a.tw = c->d_tw;
+  a.Aarr = s.dA;
+  a.Aavearr = s.dAave;
+  a.tstride = nstate(c);
+  a.fstride = s.nt;
   return a;
 }
 
-// Forward steps [s0, s0+n) on d_u0 with the stage states written to tape
-// (tape slot 0 = step s0) or, when tape == nullptr, untaped.
-int adj_forward_steps(tdhf_ctx* c, int s0, int n, zd* tape) {
+// Forward steps [s0, s0+n) of every trajectory on the batch state u0 (s.st),
+// with the stage states written to tape (slot 0 = step s0) or, when
+// tape == nullptr, untaped.  Observables of step it go to slot it.
+int adj_forward_steps(tdhf_ctx* c, const AdjState& s, int s0, int n, zd* tape) {
   const int rows = c->nb * c->nk;
+  const size_t ns = nstate(c), blk = (size_t)s.ntraj * ns;
+  zd *u0 = s.st, *ks = s.st + blk, *sa = s.st + 2 * blk, *sb = s.st + 3 * blk;
   tdhf::ApplyArgs a = base_args(c);
-  a.Aarr = c->d_A;
-  a.Aavearr = c->d_Aave;
+  a.Aarr = s.dA;
+  a.Aavearr = s.dAave;
   a.step_base = c->d_step;
-  a.obs = c->d_obs;
-  a.u0 = c->d_u0;
-  a.ksum = c->d_ksum;
+  a.obs = s.obs;
+  a.u0 = u0;
+  a.ksum = ks;
   a.tape = tape;
   a.tape_base = s0;
-  a.nstate = nstate(c);
+  a.nstate = blk;
+  a.tstride = ns;
+  a.fstride = s.nt;
+  a.ostride = (size_t)(s.nt + 1) * c->obs_stride;
   hipLaunchKernelGGL(tdhf::step_set_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, s0);
   HIPCHK(c, hipGetLastError());
   for (int j = 0; j < n; ++j) {
     a.jstep = j;
-    a.mode = tdhf::M_S1; a.u = a.w = c->d_u0; a.unext = c->d_sa;
-    CHKRET(launch_apply(c, a, rows));
-    a.mode = tdhf::M_S2; a.u = a.w = c->d_sa; a.unext = c->d_sb;
-    CHKRET(launch_apply(c, a, rows));
-    a.mode = tdhf::M_S3; a.u = a.w = c->d_sb; a.unext = c->d_sa;
-    CHKRET(launch_apply(c, a, rows));
-    a.mode = tdhf::M_S4; a.u = a.w = c->d_sa; a.unext = nullptr;
-    CHKRET(launch_apply(c, a, rows));
+    a.mode = tdhf::M_S1; a.u = a.w = u0; a.unext = sa;
+    CHKRET(launch_apply(c, a, rows, s.ntraj));
+    a.mode = tdhf::M_S2; a.u = a.w = sa; a.unext = sb;
+    CHKRET(launch_apply(c, a, rows, s.ntraj));
+    a.mode = tdhf::M_S3; a.u = a.w = sb; a.unext = sa;
+    CHKRET(launch_apply(c, a, rows, s.ntraj));
+    a.mode = tdhf::M_S4; a.u = a.w = sa; a.unext = nullptr;
+    CHKRET(launch_apply(c, a, rows, s.ntraj));
   }
   return 0;
 }
 
-// Reverse sweep over steps [s0, s0+n) whose stage states are in tape (slot 0
-// = step s0).  ub/ua/ga/gb rotate as described in tdhf_adjoint.h.
+// Reverse sweep over steps [s0, s0+n) whose stage states are in s.tape
+// (slot 0 = step s0).  ub/ua/ga/gb rotate as described in tdhf_adjoint.h.
+// The per-row field derivatives of the segment go to s.abar (step it at
+// it - s0) and are summed per stage into s.stsum.
 int adj_backward_steps(tdhf_ctx* c, AdjState& s, int s0, int n, zd** ub, zd** ua, zd** ga,
                        zd** gb) {
-  const size_t ns = nstate(c);
+  const size_t ns = nstate(c), blk = (size_t)s.ntraj * ns;
   const int rows = c->nb * c->nk;
-  tdhf::AdjArgs a = adj_base(c);
+  tdhf::AdjArgs a = adj_base(c, s);
+  a.abar_tstride = (size_t)s.seg * 4 * rows;
   for (int it = s0 + n - 1; it >= s0; --it) {
-    zd* tp = s.tape + (size_t)(it - s0) * 4 * ns;
-    const double fa[4] = {2.0 * s.Aave[it] - s.A[it], s.Aave[it], s.Aave[it], s.A[it]};
+    zd* tp = s.tape + (size_t)(it - s0) * 4 * blk;
+    a.step = it;
     a.ub = *ub;
     a.ua = *ua;
     for (int st = 4; st >= 1; --st) {
       a.mode = st;
-      a.y = tp + (size_t)(st - 1) * ns;
-      a.a = fa[st - 1];
+      a.y = tp + (size_t)(st - 1) * blk;
       a.gb = *ga;
       a.gout = *gb;
-      a.abar = s.abar + ((size_t)it * 4 + (st - 1)) * rows;
-      CHKRET(launch_adj(c, a));
+      a.abar = s.abar + ((size_t)(it - s0) * 4 + (st - 1)) * rows;
+      CHKRET(launch_adj(c, a, s.ntraj));
       std::swap(*ga, *gb);
     }
+    // current-squared objective: J after step it-1 is evaluated on y1 of step it
+    if (s.objective == TDHF_OBJ_JSQ && it > 0) CHKRET(launch_jgrad(c, s, it - 1, tp, *ua, *ga, 0));
     std::swap(*ub, *ua);  // the accumulated cotangent becomes ubar of the previous step
+  }
+  // stage sums of the segment, per trajectory: stsum[(t*nt + it)*4 + st]
+  for (int t = 0; t < s.ntraj; ++t) {
+    const long m = (long)n * 4;
+    hipLaunchKernelGGL(tdhf::row_sum_kernel, dim3((unsigned)((m + 63) / 64)), dim3(64), 0,
+                       c->stream, s.abar + (size_t)t * s.seg * 4 * rows, rows, m,
+                       s.stsum + ((size_t)t * s.nt + s0) * 4);
+    HIPCHK(c, hipGetLastError());
   }
   return 0;
 }
 
+int adj_fail_free(AdjState& s, int rc) {
+  adj_free(s);
+  return rc;
+}
+
 }  // namespace
 
-extern "C" int tdhf_fwd(tdhf_ctx* c, double* u, int nt, const double* A, double eref,
-                        double mem_budget, double* L, double* etot, double* nele, double* jtot) {
+extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const double* A,
+                              int objective, const double* eref, double mem_budget, double* L,
+                              double* etot, double* nele, double* jtot) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
-  if (!u || nt < 1 || !A) return fail(c, TDHF_EINVAL, "invalid forward arguments");
+  if (!u || nt < 1 || !A || ntraj < 1) return fail(c, TDHF_EINVAL, "invalid forward arguments");
+  if (objective != TDHF_OBJ_ENERGY && objective != TDHF_OBJ_JSQ)
+    return fail(c, TDHF_EINVAL, "unknown objective %d", objective);
   if (c->nb != c->nbocc)
     return fail(c, TDHF_EINVAL, "the adjoint path propagates occupied bands only (NB must equal NBocc)");
   AdjState& s = adj_states()[c];
   adj_free(s);
   s.nt = nt;
-  s.A.assign(A, A + nt);
-  s.Aave.resize(nt);
-  for (int it = 0; it < nt; ++it) s.Aave[it] = 0.5 * (A[it] + (it > 0 ? A[it - 1] : 0.0));
-  const size_t ns = nstate(c), sb = ns * sizeof(zd);
+  s.ntraj = ntraj;
+  s.objective = objective;
+  s.A.assign(A, A + (size_t)ntraj * nt);
+  s.Aave.resize((size_t)ntraj * nt);
+  for (int t = 0; t < ntraj; ++t)
+    for (int it = 0; it < nt; ++it) {
+      const double* At = A + (size_t)t * nt;
+      s.Aave[(size_t)t * nt + it] = 0.5 * (At[it] + (it > 0 ? At[it - 1] : 0.0));
+    }
+  const size_t ns = nstate(c), blk = (size_t)ntraj * ns, bb = blk * sizeof(zd);
+  const int rows = c->nb * c->nk;
   double budget = mem_budget;
   if (!(budget > 0.0)) {
     size_t fr = 0, tot = 0;
     HIPCHK(c, hipMemGetInfo(&fr, &tot));
     budget = 0.25 * (double)fr;
   }
-  if (4.0 * nt * sb <= budget) {
+  if (4.0 * nt * (double)bb <= budget) {
     s.full = true;
     s.seg = nt;
   } else {
     // sqrt(nt) spacing; if that does not fit, the spacing of least memory,
     // ceil(nt/S) + 4 S states with S ~ sqrt(nt/4)
-    auto need = [&](int S) { return ((double)((nt + S - 1) / S) + 4.0 * S) * (double)sb; };
+    auto need = [&](int S) { return ((double)((nt + S - 1) / S) + 4.0 * S) * (double)bb; };
     int S = (int)std::ceil(std::sqrt((double)nt));
     if (need(S) > budget) S = std::max(1, (int)std::ceil(std::sqrt(nt / 4.0)));
     if (need(S) > budget)
-      return fail(c, TDHF_ENOMEM, "memory budget %.3g B too small for %d steps", budget, nt);
+      return adj_fail_free(s, fail(c, TDHF_ENOMEM, "memory budget %.3g B too small for %d steps",
+                                   budget, nt));
     s.full = false;
     s.seg = S;
   }
-  s.tape_states = (size_t)4 * s.seg;
-  CHKRET(dalloc(c, &s.tape, s.tape_states * ns));
-  if (!s.full) {
-    s.ckpt_states = (size_t)((nt + s.seg - 1) / s.seg);
-    CHKRET(dalloc(c, &s.ckpt, s.ckpt_states * ns));
-  }
-  CHKRET(ensure_obs(c, (long)nt + 1));
-  CHKRET(ensure_field(c, nt));
-  HIPCHK(c, hipMemcpyAsync(c->d_A, s.A.data(), nt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_Aave, s.Aave.data(), nt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  CHKRET(upload(c, c->d_u0, u));
+  int rc = 0;
+  const long nsl = (long)nt + 1;
+  if ((rc = dalloc(c, &s.dA, s.A.size())) || (rc = dalloc(c, &s.dAave, s.A.size())) ||
+      (rc = dalloc(c, &s.st, 4 * blk)) || (rc = dalloc(c, &s.tape, (size_t)4 * s.seg * blk)) ||
+      (rc = dalloc(c, &s.obs, (size_t)ntraj * nsl * c->obs_stride)) ||
+      (rc = dalloc(c, &s.res, (size_t)ntraj * nsl * 7)))
+    return adj_fail_free(s, rc);
+  if (!s.full && (rc = dalloc(c, &s.ckpt, (size_t)((nt + s.seg - 1) / s.seg) * blk)))
+    return adj_fail_free(s, rc);
+  HIPCHK(c, hipMemcpyAsync(s.dA, s.A.data(), s.A.size() * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.dAave, s.Aave.data(), s.Aave.size() * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(s.st, u, bb, hipMemcpyHostToDevice, c->stream));
   if (s.full) {
-    CHKRET(adj_forward_steps(c, 0, nt, s.tape));
+    CHKRET(adj_forward_steps(c, s, 0, nt, s.tape));
   } else {
     for (int s0 = 0; s0 < nt; s0 += s.seg) {
-      HIPCHK(c, hipMemcpyAsync(s.ckpt + (size_t)(s0 / s.seg) * ns, c->d_u0, sb,
+      HIPCHK(c, hipMemcpyAsync(s.ckpt + (size_t)(s0 / s.seg) * blk, s.st, bb,
                                hipMemcpyDeviceToDevice, c->stream));
-      CHKRET(adj_forward_steps(c, s0, std::min(s.seg, nt - s0), nullptr));
+      CHKRET(adj_forward_steps(c, s, s0, std::min(s.seg, nt - s0), nullptr));
     }
   }
   // observables after the last step (slot nt) and the per-step reduction
-  const int rows = c->nb * c->nk;
   hipLaunchKernelGGL(tdhf::step_set_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, nt);
   HIPCHK(c, hipGetLastError());
   tdhf::ApplyArgs a = base_args(c);
   a.mode = tdhf::M_OBS;
-  a.u = a.w = c->d_u0;
-  a.Aarr = c->d_A;
-  a.Aavearr = c->d_Aave;
+  a.u = a.w = s.st;
+  a.Aarr = s.dA;
+  a.Aavearr = s.dAave;
   a.step_base = c->d_step;
   a.jstep = -1;
-  a.obs = c->d_obs;
-  CHKRET(launch_apply(c, a, rows));
-  const int nsl = nt + 1;
-  hipLaunchKernelGGL(tdhf::obs_reduce_kernel, dim3((nsl + 63) / 64), dim3(64), 0, c->stream,
-                     c->d_obs, c->obs_stride, rows, nsl, c->dv, c->vcell, c->d_res);
-  HIPCHK(c, hipGetLastError());
-  std::vector<double> res((size_t)7 * nsl);
-  HIPCHK(c, hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           c->stream));
-  CHKRET(sync(c));
-  for (int it = 0; it < nt; ++it) {
-    const double* r = &res[(size_t)7 * (it + 1)];
-    if (etot) etot[it] = r[0];
-    if (nele) nele[it] = r[1];
-    if (jtot) jtot[it] = r[2];
+  a.obs = s.obs;
+  a.tstride = ns;
+  a.fstride = nt;
+  a.ostride = (size_t)nsl * c->obs_stride;
+  CHKRET(launch_apply(c, a, rows, ntraj));
+  for (int t = 0; t < ntraj; ++t) {
+    hipLaunchKernelGGL(tdhf::obs_reduce_kernel, dim3((unsigned)((nsl + 63) / 64)), dim3(64), 0,
+                       c->stream, s.obs + (size_t)t * nsl * c->obs_stride, c->obs_stride, rows,
+                       (int)nsl, c->dv, c->vcell, s.res + (size_t)t * nsl * 7);
+    HIPCHK(c, hipGetLastError());
   }
-  s.jN = res[(size_t)7 * nt + 2];
-  if (L) *L = res[(size_t)7 * nt] - eref;
+  std::vector<double> res((size_t)ntraj * nsl * 7);
+  HIPCHK(c, hipMemcpyAsync(res.data(), s.res, res.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync(u, s.st, bb, hipMemcpyDeviceToHost, c->stream));
+  CHKRET(sync(c));
+  s.jt.resize((size_t)ntraj * nt);
+  s.nel.resize((size_t)ntraj * nt);
+  for (int t = 0; t < ntraj; ++t) {
+    double lsum = 0.0;
+    for (int it = 0; it < nt; ++it) {
+      const double* r = &res[((size_t)t * nsl + it + 1) * 7];
+      const size_t k = (size_t)t * nt + it;
+      if (etot) etot[k] = r[0];
+      if (nele) nele[k] = r[1];
+      if (jtot) jtot[k] = r[2];
+      s.jt[k] = r[2];
+      s.nel[k] = r[1];
+      lsum += c->dt * r[2] * r[2];
+    }
+    if (L) {
+      if (objective == TDHF_OBJ_ENERGY)
+        L[t] = res[((size_t)t * nsl + nt) * 7] - (eref ? eref[t] : 0.0);
+      else
+        L[t] = lsum;
+    }
+  }
+  if (objective == TDHF_OBJ_JSQ) {
+    CHKRET(dalloc(c, &s.dJ, s.jt.size()));
+    HIPCHK(c, hipMemcpyAsync(s.dJ, s.jt.data(), s.jt.size() * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    CHKRET(sync(c));
+  }
+  // the observables slots are not needed by the backward sweep
+  dfree(s.obs);
+  s.obs = nullptr;
   s.have_fwd = true;
-  return download(c, u, c->d_u0);
+  return 0;
 }
 
-extern "C" int tdhf_bwd(tdhf_ctx* c, double* dLdA, double* dLdu0) {
+extern "C" int tdhf_bwd_batch(tdhf_ctx* c, double* dLdA, double* dLdu0) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
   auto itS = adj_states().find(c);
   if (itS == adj_states().end() || !itS->second.have_fwd)
-    return fail(c, TDHF_EINVAL, "tdhf_bwd needs a preceding tdhf_fwd on this context");
+    return fail(c, TDHF_EINVAL, "the backward call needs a preceding forward call on this context");
   AdjState& s = itS->second;
   if (!dLdA) return fail(c, TDHF_EINVAL, "dLdA is NULL");
-  const size_t ns = nstate(c);
+  if (!s.full && !s.ckpt)
+    return fail(c, TDHF_EINVAL, "the checkpointed forward state was consumed by an earlier backward call");
+  const size_t ns = nstate(c), blk = (size_t)s.ntraj * ns;
   const int rows = c->nb * c->nk;
   const int nt = s.nt;
-  CHKRET(dalloc(c, &s.buf, 4 * ns));
-  CHKRET(dalloc(c, &s.abar, (size_t)nt * 4 * rows));
-  zd *ua = s.buf, *ub = s.buf + ns, *ga = s.buf + 2 * ns, *gb = s.buf + 3 * ns;
-  // dE/du at the final state (d_u0 holds it after the forward run)
-  tdhf::AdjArgs e = adj_base(c);
-  e.mode = tdhf::A_EGRAD;
-  e.y = c->d_u0;
-  e.a = s.A[nt - 1];
-  e.cout = ub;
-  e.gout = ga;
-  CHKRET(launch_adj(c, e));
+  if (!s.buf) CHKRET(dalloc(c, &s.buf, 4 * blk));
+  if (!s.abar) CHKRET(dalloc(c, &s.abar, (size_t)s.ntraj * s.seg * 4 * rows));
+  if (!s.stsum) CHKRET(dalloc(c, &s.stsum, (size_t)s.ntraj * nt * 4));
+  zd *ua = s.buf, *ub = s.buf + blk, *ga = s.buf + 2 * blk, *gb = s.buf + 3 * blk;
+  // cotangent of the final state (s.st holds it after the forward run)
+  if (s.objective == TDHF_OBJ_ENERGY) {
+    tdhf::AdjArgs e = adj_base(c, s);
+    e.mode = tdhf::A_EGRAD;
+    e.step = nt - 1;
+    e.y = s.st;
+    e.cout = ub;
+    e.gout = ga;
+    CHKRET(launch_adj(c, e, s.ntraj));
+  } else {
+    CHKRET(launch_jgrad(c, s, nt - 1, s.st, ub, ga, 1));
+  }
   if (s.full) {
     CHKRET(adj_backward_steps(c, s, 0, nt, &ub, &ua, &ga, &gb));
   } else {
     for (int s0 = ((nt - 1) / s.seg) * s.seg; s0 >= 0; s0 -= s.seg) {
       const int n = std::min(s.seg, nt - s0);
-      HIPCHK(c, hipMemcpyAsync(c->d_u0, s.ckpt + (size_t)(s0 / s.seg) * ns, ns * sizeof(zd),
+      HIPCHK(c, hipMemcpyAsync(s.st, s.ckpt + (size_t)(s0 / s.seg) * blk, blk * sizeof(zd),
                                hipMemcpyDeviceToDevice, c->stream));
-      CHKRET(adj_forward_steps(c, s0, n, s.tape));
+      CHKRET(adj_forward_steps(c, s, s0, n, s.tape));
       CHKRET(adj_backward_steps(c, s, s0, n, &ub, &ua, &ga, &gb));
     }
+    // s.st now holds the initial state: a second backward call would need the
+    // final one, so the forward record is spent
+    dfree(s.ckpt);
+    s.ckpt = nullptr;
   }
-  std::vector<double> ab((size_t)nt * 4 * rows);
-  HIPCHK(c, hipMemcpyAsync(ab.data(), s.abar, ab.size() * sizeof(double), hipMemcpyDeviceToHost,
+  std::vector<double> sts((size_t)s.ntraj * nt * 4);
+  HIPCHK(c, hipMemcpyAsync(sts.data(), s.stsum, sts.size() * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
-  if (dLdu0) HIPCHK(c, hipMemcpyAsync(dLdu0, ub, ns * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
+  if (dLdu0)
+    HIPCHK(c, hipMemcpyAsync(dLdu0, ub, blk * sizeof(zd), hipMemcpyDeviceToHost, c->stream));
   CHKRET(sync(c));
-  for (int it = 0; it < nt; ++it) dLdA[it] = 0.0;
-  for (int it = 0; it < nt; ++it) {
-    double st[4];
-    for (int k = 0; k < 4; ++k) {
-      double acc = 0.0;
-      for (int r = 0; r < rows; ++r) acc += ab[((size_t)it * 4 + k) * rows + r];
-      st[k] = acc;
+  for (int t = 0; t < s.ntraj; ++t) {
+    double* g = dLdA + (size_t)t * nt;
+    const double* st4 = sts.data() + (size_t)t * nt * 4;
+    for (int it = 0; it < nt; ++it) g[it] = 0.0;
+    for (int it = 0; it < nt; ++it) {
+      const double* st = st4 + (size_t)it * 4;
+      if (it > 0) g[it - 1] += st[0] + 0.5 * (st[1] + st[2]);
+      g[it] += st[3] + 0.5 * (st[1] + st[2]);
     }
-    if (it > 0) dLdA[it - 1] += st[0] + 0.5 * (st[1] + st[2]);
-    dLdA[it] += st[3] + 0.5 * (st[1] + st[2]);
+    if (s.objective == TDHF_OBJ_ENERGY) {
+      g[nt - 1] += -s.jt[(size_t)t * nt + nt - 1] * c->vcell;  // dEtot/dA at the final state
+    } else {
+      for (int it = 0; it < nt; ++it) {  // dJ/dA = -N/vcell after every step
+        const size_t k = (size_t)t * nt + it;
+        g[it] += 2.0 * c->dt * s.jt[k] * (-s.nel[k] / c->vcell);
+      }
+    }
   }
-  dLdA[nt - 1] += -s.jN * c->vcell;  // dEtot/dA at the final state
   return 0;
+}
+
+extern "C" int tdhf_fwd(tdhf_ctx* c, double* u, int nt, const double* A, double eref,
+                        double mem_budget, double* L, double* etot, double* nele, double* jtot) {
+  return tdhf_fwd_batch(c, 1, u, nt, A, TDHF_OBJ_ENERGY, &eref, mem_budget, L, etot, nele, jtot);
+}
+
+extern "C" int tdhf_bwd(tdhf_ctx* c, double* dLdA, double* dLdu0) {
+  return tdhf_bwd_batch(c, dLdA, dLdu0);
 }
 
 extern "C" int tdhf_adjoint_info(tdhf_ctx* c, int* full_tape, int* segment) {

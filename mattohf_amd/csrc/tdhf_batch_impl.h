@@ -159,10 +159,10 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
   }
   const size_t lds = lds_for(b->p.ng, nw);
   switch (b->p.ng) {
-    case 16: launch_apply_t<16>(a, b->ngrid, nw, lds, b->stream); break;
-    case 32: launch_apply_t<32>(a, b->ngrid, nw, lds, b->stream); break;
-    case 64: launch_apply_t<64>(a, b->ngrid, nw, lds, b->stream); break;
-    case 128: launch_apply_t<128>(a, b->ngrid, nw, lds, b->stream); break;
+    case 16: launch_apply_t<16>(a, b->ngrid, 1, nw, lds, b->stream); break;
+    case 32: launch_apply_t<32>(a, b->ngrid, 1, nw, lds, b->stream); break;
+    case 64: launch_apply_t<64>(a, b->ngrid, 1, nw, lds, b->stream); break;
+    case 128: launch_apply_t<128>(a, b->ngrid, 1, nw, lds, b->stream); break;
   }
   BHIP(b, hipGetLastError());
   if (b->timing) BHIP(b, hipEventRecord(e1, b->stream));

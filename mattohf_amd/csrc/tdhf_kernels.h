@@ -83,7 +83,13 @@ struct ApplyArgs {
   const int2* rowmap;       // batched launch: (trajectory, row) per workgroup
   zd* tape;                 // RK4 stage states y1..y4 per step (adjoint), or nullptr
   int tape_base;            // first step held by the tape
-  size_t nstate;            // NG*NB*NK (tape stride of one state)
+  size_t nstate;            // tape stride of one stage block (all trajectories)
+  // uniform batch (grid y = trajectory, same NK and tables): offsets of
+  // trajectory t are t*tstride (states, zd), t*fstride (A, Aave), t*ostride
+  // (observables)
+  size_t tstride;
+  int fstride;
+  size_t ostride;
 };
 
 __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
@@ -176,8 +182,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const double *kx = a.kx, *vion = a.vion, *vtl = a.vtl, *kc = a.kc;
   const double *Aarr = a.Aarr, *Aavearr = a.Aavearr;
   const zd *u = a.u, *w = a.w;
-  zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext;
+  zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext, *out = a.out;
   double* obs = a.obs;
+  const int tr = blockIdx.y;  // trajectory of a uniform batch (0 otherwise)
+  if (tr > 0) {
+    Aarr += (size_t)tr * a.fstride;
+    Aavearr += (size_t)tr * a.fstride;
+  }
   if (a.td) {
     const int2 rm = a.rowmap[blockIdx.x];
     if (rm.x < 0) return;  // padding entry of the schedule (whole workgroup)
@@ -247,6 +258,17 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       case M_S4: u = w = tp + 3 * a.nstate; break;
       default: break;
     }
+  }
+  if (tr > 0) {
+    const size_t so = (size_t)tr * a.tstride;
+    u += so;
+    w += so;
+    if (u0) u0 += so;
+    if (ksum) ksum += so;
+    if (unext) unext += so;
+    if (out) out += so;
+    if (tape0) tape0 += so;
+    if (obs) obs += (size_t)tr * a.ostride;
   }
 
   // ---- prologue: w of this row into LDS ----
@@ -423,7 +445,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     h = zadd(h, zscal(wv, vh + vionv));
     h = zadd(h, vf);
     if (a.mode == M_HPSI) {
-      gstore(a.out + o, h);
+      gstore(out + o, h);
     } else if (a.mode >= M_S1 && a.mode <= M_S4) {
       const zd kk = mk(h.y, -h.x);  // -i * H w
       const double hdt = 0.5 * a.dt;

@@ -61,6 +61,8 @@ SIGNATURES = {
     "tdhf_fwd": (_i, [_vp, _vp, _i, _vp, _d, _d, C.POINTER(_d), _vp, _vp, _vp]),
     "tdhf_bwd": (_i, [_vp, _vp, _vp]),
     "tdhf_adjoint_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "tdhf_fwd_batch": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _d, _vp, _vp, _vp, _vp]),
+    "tdhf_bwd_batch": (_i, [_vp, _vp, _vp]),
     "tdhf_batch_create": (_i, [_PP, _i, _vp, _vp, C.POINTER(_vp)]),
     "tdhf_batch_destroy": (_i, [_vp]),
     "tdhf_batch_last_error": (C.c_char_p, [_vp]),
@@ -335,6 +337,7 @@ class TDHF:
         _chk(load().tdhf_fwd(self.h, _ptr(u), nt, _ptr(A), float(eref), float(mem_budget),
                              C.byref(L), _ptr(e), _ptr(n), _ptr(j)), self.h)
         self._fwd_nt = nt
+        self._fwd_ntraj = 1
         return L.value, u, e, n, j
 
     def bwd(self, want_u0=True):
@@ -347,6 +350,42 @@ class TDHF:
         gu = np.empty(self.shape, dtype=np.complex128, order="F") if want_u0 else None
         _chk(load().tdhf_bwd(self.h, _ptr(g), _ptr(gu)), self.h)
         return g, gu
+
+    def fwd_batch(self, us, A, objective="energy", eref=None, mem_budget=0.0, want_obs=True):
+        """Forward pass for len(us) trajectories of this cell in one launch per
+        stage (tdhf_fwd_batch).  A: (ntraj, nt).  objective "energy"
+        (L = Etot(u_N) - eref) or "jsq" (L = dt sum_n J_n^2).
+        Returns (L[ntraj], finals[list], etot, nele, jtot) with (ntraj, nt) arrays."""
+        ntraj = len(us)
+        A = np.ascontiguousarray(np.asarray(A, dtype=np.float64).reshape(ntraj, -1))
+        nt = A.shape[1]
+        u = np.concatenate([self._u(x).ravel(order="F") for x in us])
+        obj = {"energy": 0, "jsq": 1}[objective]
+        er = None if eref is None else _f64(np.broadcast_to(np.asarray(eref, dtype=np.float64),
+                                                            (ntraj,)).copy())
+        L = np.empty(ntraj)
+        e, n, j = ((np.empty((ntraj, nt)), np.empty((ntraj, nt)), np.empty((ntraj, nt)))
+                   if want_obs else (None,) * 3)
+        _chk(load().tdhf_fwd_batch(self.h, ntraj, _ptr(u), nt, _ptr(A), obj, _ptr(er),
+                                   float(mem_budget), _ptr(L), _ptr(e), _ptr(n), _ptr(j)), self.h)
+        self._fwd_nt = nt
+        self._fwd_ntraj = ntraj
+        ns = int(np.prod(self.shape))
+        finals = [u[t * ns:(t + 1) * ns].reshape(self.shape, order="F") for t in range(ntraj)]
+        return L, finals, e, n, j
+
+    def bwd_batch(self, want_u0=True):
+        """Backward pass after fwd_batch: (dL/dA (ntraj, nt), list of dL/du0 or None)."""
+        nt, ntraj = getattr(self, "_fwd_nt", None), getattr(self, "_fwd_ntraj", 1)
+        if not nt:
+            raise TDHFError("bwd_batch needs a preceding fwd_batch")
+        g = np.empty((ntraj, nt))
+        ns = int(np.prod(self.shape))
+        gu = np.empty(ntraj * ns, dtype=np.complex128) if want_u0 else None
+        _chk(load().tdhf_bwd_batch(self.h, _ptr(g), _ptr(gu)), self.h)
+        gus = ([gu[t * ns:(t + 1) * ns].reshape(self.shape, order="F") for t in range(ntraj)]
+               if want_u0 else None)
+        return g, gus
 
     def adjoint_info(self):
         f, s = C.c_int(), C.c_int()

@@ -72,6 +72,8 @@ def lib():
             "orc_gs_run": (None, [vp, vp, i, i, vp, vp, vp]),
             "orc_adjoint_energy": (d, [vp, vp, d, i, vp, d, vp, vp]),
             "orc_objective_energy": (d, [vp, vp, d, i, vp, d]),
+            "orc_adjoint_jsq": (d, [vp, vp, d, i, vp, vp, vp]),
+            "orc_objective_jsq": (d, [vp, vp, d, i, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -261,6 +263,19 @@ class Oracle:
         gu = np.empty(self.sys.shape, dtype=np.complex128, order="F")
         L = lib().orc_adjoint_energy(self.h, _p(self._u(u0)), float(dt), A.size, _p(A),
                                      float(eref), _p(g), _p(gu))
+        return L, g, gu
+
+    def objective_jsq(self, u0, dt, A):
+        """L = dt * sum_it J(u_{it+1}, A[it])^2 over len(A) RK4 steps (Aave from A)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        return lib().orc_objective_jsq(self.h, _p(self._u(u0)), float(dt), A.size, _p(A))
+
+    def adjoint_jsq(self, u0, dt, A):
+        """(L, dL/dA[0..N-1], dL/du0) for L = dt * sum_it J_it^2."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        g = np.empty(A.size)
+        gu = np.empty(self.sys.shape, dtype=np.complex128, order="F")
+        L = lib().orc_adjoint_jsq(self.h, _p(self._u(u0)), float(dt), A.size, _p(A), _p(g), _p(gu))
         return L, g, gu
 
     def GS_calculation(self, u, nscf, ncg):

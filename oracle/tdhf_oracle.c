@@ -935,16 +935,41 @@ static double energy_grad(const orc_ctx* c, const zc* u, double A, zc* cu) {
   return -orc_je(c, (const double*)u, A) * c->vcell;
 }
 
-double orc_adjoint_energy(const orc_ctx* c, const double* u0_, double dt, int nt, const double* A,
-                          double eref, double* dLdA, double* dLdu0) {
+/* cotangent of J(u, A) (ubk_Je, src/Functions.f90:289-318) with respect to u:
+ * J = -(dv/vcell) sum occ Re<u, P_A u>, so cj = -(2 occ dv / vcell) P_A u;
+ * returns dJ/dA = -N/vcell (N = sum occ |u|^2 dv). */
+static double current_grad(const orc_ctx* c, const zc* u, double A, zc* cj) {
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk;
+  zc* t1 = malloc(sizeof(zc) * NG);
+  for (int ik = 0; ik < NK; ++ik)
+    for (int ib = 0; ib < NB; ++ib) {
+      const size_t o = (size_t)NG * (ib + NB * (size_t)ik);
+      pvel(c, ik, A, u + o, cj + o, t1);
+      const double f = -2.0 * c->occ[ib + NB * ik] * c->dvcell / c->vcell;
+      for (int g = 0; g < NG; ++g) cj[o + g] *= f;
+    }
+  free(t1);
+  return -orc_nele(c, (const double*)u) / c->vcell;
+}
+
+/* Reverse-mode gradient of the objective after nt RK4 steps (RT_RK4_ubk,
+ * src/RT_calculation.f90:154-173) driven by A[0..nt-1]:
+ *   kind 0: L = Etot(u_nt, A[nt-1]) - eref
+ *   kind 1: L = dt sum_{it<nt} J(u_{it+1}, A[it])^2   (J after every step)
+ * Stores every stage state; returns L, fills dLdA[nt] and optionally dLdu0. */
+static double adjoint_common(const orc_ctx* c, const double* u0_, double dt, int nt,
+                             const double* A, int kind, double eref, double* dLdA,
+                             double* dLdu0) {
   const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
   zc* tape = (zc*)malloc(sizeof(zc) * n * 4 * (size_t)nt);
   zc* u = (zc*)malloc(sizeof(zc) * n);
   zc *h = malloc(sizeof(zc) * n), *kk = malloc(sizeof(zc) * n), *ks = malloc(sizeof(zc) * n);
   double* Aave = (double*)malloc(sizeof(double) * nt);
+  double* jt = (double*)malloc(sizeof(double) * nt);
   for (int it = 0; it < nt; ++it) Aave[it] = 0.5 * (A[it] + (it > 0 ? A[it - 1] : 0.0));
   memcpy(u, u0_, sizeof(zc) * n);
   const double hdt = 0.5 * dt;
+  double L = 0.0;
   for (int it = 0; it < nt; ++it) { /* forward, storing y1..y4 */
     const double a[4] = {2.0 * Aave[it] - A[it], Aave[it], Aave[it], A[it]};
     const double cc[4] = {hdt, hdt, dt, 0.0};
@@ -959,14 +984,25 @@ double orc_adjoint_energy(const orc_ctx* c, const double* u0_, double dt, int nt
         for (size_t i = 0; i < n; ++i) y[n * (s + 1) + i] = u[i] + cc[s] * kk[i];
     }
     for (size_t i = 0; i < n; ++i) u[i] = u[i] + ks[i] * dt / 6.0;
+    if (kind == 1) {
+      jt[it] = orc_je(c, (const double*)u, A[it]);
+      L += dt * jt[it] * jt[it];
+    }
   }
-  const double L = orc_etot(c, (double*)u, A[nt - 1]) - eref;
+  if (kind == 0) L = orc_etot(c, (double*)u, A[nt - 1]) - eref;
   for (int it = 0; it < nt; ++it) dLdA[it] = 0.0;
   zc* ub = (zc*)malloc(sizeof(zc) * n);
   zc* un = (zc*)malloc(sizeof(zc) * n);
   zc* kb = (zc*)malloc(sizeof(zc) * n);
   zc* vj = (zc*)malloc(sizeof(zc) * n);
-  dLdA[nt - 1] += energy_grad(c, u, A[nt - 1], ub);
+  if (kind == 0) {
+    dLdA[nt - 1] += energy_grad(c, u, A[nt - 1], ub);
+  } else {
+    const double dj = current_grad(c, u, A[nt - 1], ub);
+    const double w = 2.0 * dt * jt[nt - 1];
+    for (size_t i = 0; i < n; ++i) ub[i] *= w;
+    dLdA[nt - 1] += w * dj;
+  }
   for (int it = nt - 1; it >= 0; --it) {
     const double a[4] = {2.0 * Aave[it] - A[it], Aave[it], Aave[it], A[it]};
     const double bw[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
@@ -989,9 +1025,42 @@ double orc_adjoint_energy(const orc_ctx* c, const double* u0_, double dt, int nt
     if (it > 0) dLdA[it - 1] += abar[0] + 0.5 * (abar[1] + abar[2]);
     dLdA[it] += abar[3] + 0.5 * (abar[1] + abar[2]);
     memcpy(ub, un, sizeof(zc) * n);
+    if (kind == 1 && it > 0) { /* J after step it-1 is evaluated on y1 of step it */
+      const double dj = current_grad(c, y, A[it - 1], vj);
+      const double w = 2.0 * dt * jt[it - 1];
+      for (size_t i = 0; i < n; ++i) ub[i] += w * vj[i];
+      dLdA[it - 1] += w * dj;
+    }
   }
   if (dLdu0) memcpy(dLdu0, ub, sizeof(zc) * n);
-  free(ub); free(un); free(kb); free(vj); free(tape); free(u); free(h); free(kk); free(ks); free(Aave);
+  free(ub); free(un); free(kb); free(vj); free(tape); free(u); free(h); free(kk); free(ks);
+  free(Aave); free(jt);
+  return L;
+}
+
+double orc_adjoint_energy(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
+                          double eref, double* dLdA, double* dLdu0) {
+  return adjoint_common(c, u0, dt, nt, A, 0, eref, dLdA, dLdu0);
+}
+
+double orc_adjoint_jsq(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
+                       double* dLdA, double* dLdu0) {
+  return adjoint_common(c, u0, dt, nt, A, 1, 0.0, dLdA, dLdu0);
+}
+
+/* dt sum_{it<nt} J(u_{it+1}, A[it])^2 alone (for finite differences). */
+double orc_objective_jsq(const orc_ctx* c, const double* u0, double dt, int nt, const double* A) {
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  double* u = (double*)malloc(sizeof(zc) * n);
+  memcpy(u, u0, sizeof(zc) * n);
+  double L = 0.0;
+  for (int it = 0; it < nt; ++it) {
+    const double aave = 0.5 * (A[it] + (it > 0 ? A[it - 1] : 0.0));
+    orc_rk4_step(c, u, dt, aave, A[it]);
+    const double j = orc_je(c, u, A[it]);
+    L += dt * j * j;
+  }
+  free(u);
   return L;
 }
 

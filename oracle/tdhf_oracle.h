@@ -75,6 +75,10 @@ double orc_adjoint_energy(const orc_ctx* c, const double* u0, double dt, int nt,
                           double eref, double* dLdA, double* dLdu0);
 double orc_objective_energy(const orc_ctx* c, const double* u0, double dt, int nt,
                             const double* A, double eref);
+/* L = dt * sum_{it<nt} J(u_{it+1}, A[it])^2 (time-integrated current squared) */
+double orc_adjoint_jsq(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
+                       double* dLdA, double* dLdu0);
+double orc_objective_jsq(const orc_ctx* c, const double* u0, double dt, int nt, const double* A);
 
 #ifdef __cplusplus
 }

@@ -98,3 +98,87 @@ def test_forward_matches_rt_run():
     np.testing.assert_array_equal(e, e1)
     np.testing.assert_array_equal(j, j1)
     np.testing.assert_array_equal(uf, u1)
+
+
+# ---- current-squared objective and many trajectories per launch ----
+
+@pytest.mark.parametrize("case", CASES[:2], ids=lambda c: f"ng{c[0]}_nk{c[1]}")
+def test_jsq_matches_oracle(case):
+    """L = dt sum_n J_n^2: GPU pair vs the oracle's adjoint (pinned by finite
+    differences in test_oracle.py)."""
+    ng, nk, nb, a, z, nt, dt = case
+    S, O, vion, u0, A = setup_case(*case)
+    Lo, go, guo = O.adjoint_jsq(u0, dt, A)
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        L, _, _, _, j = G.fwd_batch([u0], A[None, :], objective="jsq")
+        g, gu = G.bwd_batch()
+    assert abs(L[0] - Lo) < 1e-11 * max(1.0, abs(Lo))
+    assert np.abs(g[0] - go).max() < 1e-9 * max(1.0, np.abs(go).max())
+    assert np.abs(gu[0] - guo).max() < 1e-9 * max(1.0, np.abs(guo).max())
+    assert abs(L[0] - dt * np.sum(j[0] ** 2)) < 1e-14 * max(1.0, abs(L[0]))
+
+
+def test_jsq_finite_differences():
+    ng, nk, nb, a, z, nt, dt = CASES[1]
+    S, O, vion, u0, A = setup_case(*CASES[1])
+    rng = np.random.default_rng(11)
+    h = 1e-5
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        G.fwd_batch([u0], A[None, :], objective="jsq")
+        g, gu = G.bwd_batch()
+        f = lambda uu, AA: G.fwd_batch([uu], AA[None, :], objective="jsq", want_obs=False)[0][0]
+        for _ in range(3):
+            d = rng.standard_normal(nt)
+            d /= np.linalg.norm(d)
+            fd = (f(u0, A + h * d) - f(u0, A - h * d)) / (2 * h)
+            assert abs(fd - g[0] @ d) <= 1e-6 * abs(fd) + 1e-13, (fd, g[0] @ d)
+        du = rng.standard_normal(S.shape) + 1j * rng.standard_normal(S.shape)
+        du /= np.linalg.norm(du)
+        fd = (f(u0 + h * du, A) - f(u0 - h * du, A)) / (2 * h)
+        assert abs(fd - np.real(np.vdot(gu[0], du))) <= 1e-6 * abs(fd) + 1e-13
+
+
+@pytest.mark.parametrize("objective", ["energy", "jsq"])
+def test_batch_equals_single_runs(objective):
+    """Three trajectories (own pulse, own initial state) in one launch per stage
+    give each trajectory's single-run objective and gradient."""
+    ng, nk, nb, a, z, nt, dt = CASES[1]
+    S, O, vion, u0, A0 = setup_case(*CASES[1])
+    rng = np.random.default_rng(5)
+    As = np.stack([A0, 0.1 * rng.standard_normal(nt), 0.05 * rng.standard_normal(nt)])
+    us = [u0, S.gram_schmidt(u0 + 0.05 * rng.standard_normal(S.shape)), u0]
+    er = np.array([0.1, -0.2, 0.3])
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        L, fin, e, n, j = G.fwd_batch(us, As, objective=objective, eref=er)
+        g, gu = G.bwd_batch()
+        for t in range(3):
+            L1, fin1, e1, n1, j1 = G.fwd_batch([us[t]], As[t:t + 1], objective=objective,
+                                               eref=er[t:t + 1])
+            g1, gu1 = G.bwd_batch()
+            assert abs(L[t] - L1[0]) <= 1e-13 * max(1.0, abs(L1[0]))
+            np.testing.assert_allclose(j[t], j1[0], rtol=0, atol=1e-13 * np.abs(j1).max())
+            np.testing.assert_allclose(fin[t], fin1[0], rtol=0, atol=1e-13)
+            np.testing.assert_allclose(g[t], g1[0], rtol=0, atol=1e-12 * np.abs(g1).max())
+            np.testing.assert_allclose(gu[t], gu1[0], rtol=0, atol=1e-12 * np.abs(gu1[0]).max())
+
+
+def test_batch_jsq_checkpointed_equals_full_tape():
+    ng, nk, nb, a, z, nt, dt = 32, 4, 2, 8.0, 4.0, 29, 0.025
+    S, O, vion, u0, A0 = setup_case(ng, nk, nb, a, z, nt, dt)
+    rng = np.random.default_rng(9)
+    As = np.stack([A0, 0.1 * rng.standard_normal(nt)])
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        L1, _, _, _, _ = G.fwd_batch([u0, u0], As, objective="jsq")
+        assert G.adjoint_info()[0]
+        g1, gu1 = G.bwd_batch()
+        state = 2 * ng * nb * nk * 16
+        L2, _, _, _, _ = G.fwd_batch([u0, u0], As, objective="jsq", mem_budget=30 * state)
+        full, seg = G.adjoint_info()
+        assert not full and 1 < seg < nt
+        g2, gu2 = G.bwd_batch()
+        with pytest.raises(T.TDHFError):
+            G.bwd_batch()  # the checkpointed record is spent
+    np.testing.assert_array_equal(L1, L2)
+    np.testing.assert_allclose(g2, g1, rtol=0, atol=1e-14 * np.abs(g1).max())
+    for a_, b_ in zip(gu2, gu1):
+        np.testing.assert_allclose(a_, b_, rtol=0, atol=1e-14 * np.abs(b_).max())

@@ -27,6 +27,9 @@ function is therefore checked against something independent of its code:
   orc_ks_krylov      Ritz subspace invariant under rotation of the input bands
   orc_gs_run         converged residuals; eigenvalues independent of NCG
   orc_define_afield  closed-form sin^2 pulse
+  orc_adjoint_*      central finite differences of the oracle's own forward
+                     objective (energy at the end and dt*sum J^2), in the
+                     field and in the initial orbitals
 """
 import math
 
@@ -364,3 +367,33 @@ def test_afield_lp_closed_form():
     np.testing.assert_allclose(A, ref, rtol=0, atol=1e-12)
     np.testing.assert_allclose(Aave[1:], 0.5 * (A[1:] + A[:-1]), rtol=0, atol=0)
     np.testing.assert_allclose(E[1:], -(A[1:] - A[:-1]) / dt, rtol=1e-15, atol=0)
+
+
+def _adjoint_case(seed=3):
+    S = po.System(16, 2, 1, 1, 5.0)
+    vion = S.ion_potential([2.0], [1.0], [0.0])
+    O = po.Oracle(S, vion)
+    u0 = S.gram_schmidt(S.init_wf())
+    rng = np.random.default_rng(seed)
+    A = 0.2 * rng.standard_normal(6)
+    return S, O, u0, A, rng
+
+
+@pytest.mark.parametrize("kind", ["energy", "jsq"])
+def test_oracle_adjoint_finite_differences(kind):
+    S, O, u0, A, rng = _adjoint_case()
+    dt, h = 0.02, 1e-5
+    if kind == "energy":
+        L, g, gu = O.adjoint_energy(u0, dt, A, eref=0.0)
+        f = lambda uu, AA: O.objective_energy(uu, dt, AA)
+    else:
+        L, g, gu = O.adjoint_jsq(u0, dt, A)
+        f = lambda uu, AA: O.objective_jsq(uu, dt, AA)
+    assert abs(L - f(u0, A)) < 1e-13 * max(1.0, abs(L))
+    for _ in range(3):
+        d = rng.standard_normal(A.size)
+        fd = (f(u0, A + h * d) - f(u0, A - h * d)) / (2 * h)
+        assert abs(fd - g @ d) <= 1e-6 * abs(fd) + 1e-11, (fd, g @ d)
+    du = rng.standard_normal(S.shape) + 1j * rng.standard_normal(S.shape)
+    fd = (f(u0 + h * du, A) - f(u0 - h * du, A)) / (2 * h)
+    assert abs(fd - np.real(np.vdot(gu, du))) <= 1e-6 * abs(fd) + 1e-11
