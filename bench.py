@@ -271,6 +271,86 @@ def adjoint_bench(seed=4, cpu_check=True):
     return out
 
 
+SCALE_CFG = dict(acell=12.0, ng=128, nk=128, nb=8, nbocc=8, z=16.0, alpha=1.0, dt=0.004,
+                 ntraj=32, nt_full=8000, T=400.0, fd_h=1e-5)
+
+
+def adjoint_scale_bench(nt=8, seed=20261205, gs_iters=40):
+    """Many-pulse adjoint at scale: a=12, NG=128, NK=128, NB=NBocc=8 (Z=16), 32
+    pulses (A0 log-uniform [0.01, 0.5], omega uniform [0.03, 0.10] Ha, CEP
+    uniform, sin^2 envelope T=400 au), objective L = dt sum_n J_n^2, gradients
+    of all 32 pulses per forward/backward pair.  A bounded sample of nt steps
+    is timed (the specified 8000 steps take ~1e3 x longer); the memory budget
+    is set so the sample uses sqrt(nt) checkpoints, the spacing the full run
+    uses under its budget.  dt = 0.004 au: 0.02 au exceeds the RK4 stability
+    bound of this grid; the per-step cost does not depend on dt.
+    Reports the plain batched forward rate (tdhf_batch, same pulses), the
+    gradient cost relative to it, and central finite differences of every
+    pulse's L along one random direction each (one pair of batched forwards)."""
+    from mattohf_amd import tdhf as T
+
+    c = SCALE_CFG
+    ng, nk, nb, a, dt, ntr = c["ng"], c["nk"], c["nb"], c["acell"], c["dt"], c["ntraj"]
+    vion = T.ion_potential(ng, nk, a, [c["z"]], [c["alpha"]], [0.0])
+    G = T.TDHF(ng, nk, nb, c["nbocc"], a, dt, vion)
+    t0 = time.perf_counter()
+    u = G.Gram_Schmidt(T.init_wf(ng, nb, nk, a))
+    ugs = G.GS_calculation(u, gs_iters, 4, tol=1e-11)[0]
+    t_gs = time.perf_counter() - t0
+    A, Aave = batch_pulses(ntr, nt, dt, c["T"], seed)
+    us = [ugs] * ntr
+    # plain batched forward of the same pulses (E/N/J every step)
+    B = T.Batch(ng, nb, c["nbocc"], a, dt, [nk] * ntr, np.array([vion] * ntr))
+    B.upload(us)
+    B.propagate(A[:, :2], Aave[:, :2])  # warm-up (the state is re-uploaded)
+    B.upload(us)
+    t0 = time.perf_counter()
+    B.propagate(A, Aave)
+    t_plain = time.perf_counter() - t0
+    B.close()
+    state = ng * nb * nk * 16 * ntr
+    S = int(math.ceil(math.sqrt(nt)))
+    budget = ((nt + S - 1) // S + 4 * S) * state
+    G.fwd_batch(us, A[:, :2], objective="jsq", want_obs=False)  # warm-up
+    t0 = time.perf_counter()
+    L, _, _, _, j = G.fwd_batch(us, A, objective="jsq", mem_budget=budget)
+    t1 = time.perf_counter()
+    g, _ = G.bwd_batch(want_u0=False)
+    t2 = time.perf_counter()
+    full, seg = G.adjoint_info()
+    rng = np.random.default_rng(seed + 1)
+    d = rng.standard_normal(A.shape)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    h = c["fd_h"]
+    lp = G.fwd_batch(us, A + h * d, objective="jsq", want_obs=False)[0]
+    lm = G.fwd_batch(us, A - h * d, objective="jsq", want_obs=False)[0]
+    G.close()
+    fd = (lp - lm) / (2 * h)
+    ad = np.einsum("tn,tn->t", g, d)
+    rel = np.abs(fd - ad) / np.maximum(np.abs(fd), 1e-300)
+    S_full = int(math.ceil(math.sqrt(c["nt_full"])))
+    plan_bytes = ((c["nt_full"] + S_full - 1) // S_full + 4 * S_full) * state
+    fstep = flops_per_step(ng, nk, nb, c["nbocc"])
+    plain_rate = ntr * nt / t_plain
+    return {
+        "workload": "a=12, NG=128, NK=128, NB=8, 32 pulses per launch, objective dt*sum J^2, "
+                    f"{nt}-step sample of the {c['nt_full']}-step run, dt={dt}",
+        "gs_seconds": t_gs,
+        "plain_batched_traj_steps_per_s": plain_rate,
+        "plain_batched_achieved_tflops": plain_rate * fstep / 1e12,
+        "plain_batched_frac": plain_rate * fstep / 1e12 / PEAK_FP64_TFLOPS,
+        "t_fwd_taped_s": t1 - t0,
+        "t_bwd_s": t2 - t1,
+        "gradient_over_forward": (t2 - t0) / t_plain,
+        "tape": "full" if full else f"sqrt(N) checkpoints every {seg} steps",
+        "full_run_checkpoint_plan": {"spacing": S_full, "bytes": plan_bytes},
+        "fd_h": h,
+        "fd_max_rel_err": float(rel.max()),
+        "fd_pulses": int(ntr),
+        "L_range": [float(L.min()), float(L.max())],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -283,6 +363,10 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=30)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-adjoint-scale", action="store_true")
+    ap.add_argument("--adjoint-scale-steps", type=int, default=8)
+    ap.add_argument("--only", choices=["adjoint-scale"], default=None,
+                    help="run one secondary leg alone and print its JSON")
     ap.add_argument("--gs-iters", type=int, default=60,
                     help="ground-state iterations before the timed run (0: orthonormalised "
                          "initial guess; used for profiling runs)")
@@ -299,6 +383,10 @@ def main():
 
     from mattohf_amd import tdhf as T
 
+    if args.only == "adjoint-scale":
+        print(json.dumps({"adjoint_scale": adjoint_scale_bench(nt=args.adjoint_scale_steps)}),
+              flush=True)
+        return
     cfg = dict(CFG)
     G, vion, u, eh = setup(cfg, gs_iters=args.gs_iters)
     nt = args.warmup + args.steps + args.timing_steps
@@ -381,6 +469,8 @@ def main():
         }
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(cpu_check=not args.no_cpu)
+    if not args.no_adjoint_scale:
+        out["adjoint_scale"] = adjoint_scale_bench(nt=args.adjoint_scale_steps)
     if not args.no_cpu:
         G2 = setup(cfg, gs_iters=0)[0]
         out["cpu_baseline"] = cpu_baseline(cfg, vion, G2, u, A, Aave, budget_s=args.cpu_seconds)

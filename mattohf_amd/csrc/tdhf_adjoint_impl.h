@@ -77,6 +77,8 @@ size_t adj_lds_for(int ng, int nw) {
 int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
   const int rows = c->nb * c->nk;
   int nw = choose_nw(c->ng, c->nk, c->nb);
+  // fewer waves where the tiles would exceed the 160 KiB of LDS (NG = 128)
+  while (nw > (c->ng + 63) / 64 && adj_lds_for(c->ng, nw) > 160 * 1024) --nw;
   const size_t lds = adj_lds_for(c->ng, nw);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {

@@ -19,6 +19,7 @@ CASES = [  # ng, nk, nb, a, z, nt, dt
     (16, 3, 2, 5.0, 2.0, 12, 0.02),
     (32, 4, 2, 8.0, 4.0, 30, 0.025),
     (64, 3, 2, 10.0, 8.0, 20, 0.01),
+    (128, 2, 2, 12.0, 4.0, 6, 0.004),
 ]
 
 
