@@ -41,7 +41,7 @@ struct tdhf_ctx {
   std::string err = "no error";
   std::vector<double> kx;
   // device tables
-  double *d_kx = nullptr, *d_vion = nullptr, *d_vcg = nullptr, *d_vcgl = nullptr;
+  double *d_kx = nullptr, *d_vion = nullptr, *d_vcg = nullptr;
   double *d_vq = nullptr, *d_vtl = nullptr;
   double* d_kc = nullptr;  // circulant kernels K_H, K_2, K_1 of the apply kernel
   zd* d_tw = nullptr;
@@ -463,7 +463,6 @@ int build_tables(tdhf_ctx* c, const double* vcg, const double* vcgkk) {
     HIPCHK(c, hipMemcpy(c->d_vq, rows.data(), rows.size() * sizeof(double),
                         hipMemcpyHostToDevice));
   }
-  CHKRET(dalloc(c, &c->d_vcgl, NG));
   CHKRET(dalloc(c, &c->d_vtl, (size_t)NG * ND));
   CHKRET(dalloc(c, &c->d_tw, 2 * NG));
   switch (NG) {
@@ -473,7 +472,6 @@ int build_tables(tdhf_ctx* c, const double* vcg, const double* vcgkk) {
     case 128: hipLaunchKernelGGL(tdhf::twiddle_kernel<128>, dim3(1), dim3(256), 0, c->stream, c->d_tw); break;
   }
   HIPCHK(c, hipGetLastError());
-  CHKRET(relayout(c, c->d_vcg, 1, 1.0 / (double)NG, c->d_vcgl));
   {
     // circulant kernels of the Hartree and kinetic round trips:
     // K_H(d) = 1/NG sum_g vCG(g) cos(2 pi g d/NG), K_2(d) = 1/NG sum_g G^2/2
@@ -510,7 +508,7 @@ void release(tdhf_ctx* c) {
   destroy_graph(c);
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   c->ev_pool.clear();
-  dfree(c->d_kx); dfree(c->d_vion); dfree(c->d_vcg); dfree(c->d_vcgl);
+  dfree(c->d_kx); dfree(c->d_vion); dfree(c->d_vcg);
   dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw); dfree(c->d_kc);
   dfree(c->d_u0); dfree(c->d_ksum); dfree(c->d_sa); dfree(c->d_sb);
   dfree(c->d_w); dfree(c->d_out);

@@ -34,8 +34,8 @@ struct AdjArgs {
   int step;
   const double* kx;
   const double* vion;
-  const double* vcgl;
   const double* vtl;
+  const double* kc;         // circulant kernels K_H, K_2, K_1 (see apply_kernel)
   const zd* tw;
   const zd* y;              // stage state (all rows)
   const zd* gb;             // cotangent gbar of H y (all rows); A_EGRAD: unused
@@ -45,6 +45,14 @@ struct AdjArgs {
   zd* cout;                 // A_EGRAD: dE/du (Euclidean cotangent)
   double* abar;             // per-row field derivative of this stage, or nullptr
   int mode;
+  // graph-replayable form: step = *step_base - jstep, y = stage block of that
+  // step in the tape (slot step % tape_seg, blocks of blk states), abar the
+  // matching slot of the per-row derivatives (4 * rows per step)
+  const int* step_base;
+  int jstep;
+  const zd* tape;
+  int tape_seg;
+  size_t blk;
   // uniform batch (grid y = trajectory): offsets of trajectory t are
   // t*tstride (states, zd), t*fstride (fields), t*abar_tstride (abar)
   size_t tstride;
@@ -52,11 +60,16 @@ struct AdjArgs {
   size_t abar_tstride;
 };
 
+// LDS of adjoint_kernel<NG> with nw waves (bytes): the groups' tiles (pair
+// transposes, then reduction slots, circulant-product partials and the field
+// derivative staging), y, gbar, the exchange sum and K_1 (complex NG each),
+// rho, s, K_H, K_2 (NG doubles each), a small scratch and the group_reduce
+// partials.
 template <int NG>
 __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
-         5 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
-         ((size_t)64 * nw + 2 * NG) * sizeof(double);
+         4 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
+         ((size_t)64 * nw > 2 * NG ? (size_t)64 * nw : 2 * NG) * sizeof(double);
 }
 
 template <int NG>
@@ -64,6 +77,12 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   // per-trajectory view of the arguments
   AdjArgs a = a0;
   const int tr = blockIdx.y;
+  if (a0.step_base) {
+    a.step = *a0.step_base - a0.jstep;
+    const size_t slot = (size_t)(a.step % a0.tape_seg) * 4 + (a0.mode - 1);
+    a.y = a0.tape + slot * a0.blk;
+    if (a.abar) a.abar = a0.abar + slot * (size_t)(a0.nk * a0.nb);
+  }
   {
     const size_t so = (size_t)tr * a0.tstride;
     a.y += so;
@@ -97,26 +116,30 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   zd* sY = smem + ngrp * F::TSZ;   // y_ki
   zd* sG = sY + NG;                // gbar_ki
   zd* sACC = sG + NG;              // exchange part of J^T gbar
-  zd* sTG = sACC + NG;             // T gbar
-  zd* sYH = sTG + NG;              // FFT(y_ki) in G layout
-  double* sRho = reinterpret_cast<double*>(sYH + NG);
+  zd* sK1 = sACC + NG;
+  double* sRho = reinterpret_cast<double*>(sK1 + NG);
   double* sS = sRho + NG;
-  double* sVH = sS + NG;
-  double* sKS = sVH + NG;          // K s (Hartree response)
-  double* sRed = sKS + NG;
-  double* sPart = sRed + 16;  // blockDim.x + 2*NG doubles for group_reduce
+  double* sKH = sS + NG;
+  double* sK2 = sKH + NG;
+  double* sRed = sK2 + NG;
+  double* sPart = sRed + 16;  // max(blockDim.x, 2 NG) doubles for group_reduce
+  static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * F::TSZ * GPW * (NG > 64 ? NG / 64 : 1),
+                "tile area too small for the epilogue scratch");
 
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   zd yv = mk(0.0, 0.0), gv = mk(0.0, 0.0);
   double vionv = 0.0;
   if (tid < NG) {
-    yv = a.y[o];
-    gv = gbar[o];
+    yv = gload(a.y + o);
+    gv = gload(gbar + o);
     sY[tid] = yv;
     sG[tid] = gv;
-    vionv = a.vion[tid];
+    vionv = gload(a.vion + tid);
+    sKH[tid] = gload(a.kc + tid);
+    sK2[tid] = gload(a.kc + NG + tid);
+    sK1[tid] = gload(reinterpret_cast<const zd*>(a.kc + 2 * NG) + tid);
   }
-  const double kxv = a.kx[ik];
+  const double kxv = gload(a.kx + ik);
   F fft;
   fft.init(a.tw, l);
   const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = L*m + l, L even
@@ -130,7 +153,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
     den[m] = 0.0;
     sv[m] = 0.0;
   });
-  __syncthreads();
+  lds_barrier();
 
   // ---- pair loop over all partner rows (q, j) ----
   const int nitem = a.nk * a.nb;
@@ -141,19 +164,19 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
     zd uq[R], gq[R], v[R];
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      uq[m] = a.y[prow + L * m + l];
-      gq[m] = egrad ? mk(0.0, 0.0) : a.gb[prow + L * m + l];
+      uq[m] = gload(a.y + prow + L * m + l);
+      gq[m] = egrad ? mk(0.0, 0.0) : gload(a.gb + prow + L * m + l);
       den[m] += p_occ ? zabs2(uq[m]) : 0.0;
       sv[m] += gq[m].x * uq[m].x + gq[m].y * uq[m].y;
     });
     const double* vt = a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + l * R;
     // D_kq = vt_qk(NG/2) - vt_kq(NG/2): G-layout position L/2 of rows (q-k), (k-q)
-    const double dnyq = a.vtl[(size_t)(iq - ik + a.nk - 1) * NG + JNYQ] -
-                        a.vtl[(size_t)(ik - iq + a.nk - 1) * NG + JNYQ];
+    const double dnyq = gload(a.vtl + (size_t)(iq - ik + a.nk - 1) * NG + JNYQ) -
+                        gload(a.vtl + (size_t)(ik - iq + a.nk - 1) * NG + JNYQ);
     double vtr[R];
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      vtr[j] = vt[j];
+      vtr[j] = gload(vt + j);
     });
     // T1: phi' = C'_kq(y_ki conj y_qj); adds phi' gbar_qj (or, for dE/du,
     // (phi + phi') y_qj)
@@ -223,107 +246,121 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
     slotd[2 * (L * m + l)] = acc[m].x;
     slotd[2 * (L * m + l) + 1] = acc[m].y;
   });
-  __syncthreads();
+  lds_barrier();
   group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
                reinterpret_cast<double*>(sACC), 1.0);
-  __syncthreads();
+  lds_barrier();
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     slotd[L * m + l] = den[m];
     slotd[NG + L * m + l] = sv[m];
   });
-  __syncthreads();
+  lds_barrier();
   group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart, sRho, 1.0);
-  __syncthreads();
-  for (int idx = tid; idx < NG; idx += blockDim.x) sRho[idx] *= a.occ;  // sS follows sRho
-  __syncthreads();
+  lds_barrier();
 
-  // ---- wave 0: Hartree(rho) | T gbar | Hartree(s) | FFT(y_ki) ----
-  double vcg[R];
-  static_for<0, R>([&](auto jc) {
-    constexpr int j = decltype(jc)::value;
-    vcg[j] = a.vcgl[l * R + j];
-  });
-  if (wave == 0) {
-    zd v[R];
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      const int x = L * m + l;
-      v[m] = gw == 0 ? mk(sRho[x], 0.0)
-           : gw == 1 ? sG[x]
-           : gw == 2 ? mk(sS[x], 0.0)
-           : gw == 3 ? sY[x]
-                     : mk(0.0, 0.0);
-    });
-    fft.fwd(v, T, l);
-    if (gw == 3) {
-      static_for<0, R>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        sYH[l * R + j] = v[j];
-      });
-    }
-    wave_sync();
-    double pa = 0.0;
-    const double kA = kxv + fa, kA2 = kA * kA;
-    static_for<0, R>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      const double G = gvalue(F::gindex(l, j), NG, a.bx);
-      if (gw == 0 || gw == 2) {
-        v[j] = zscal(v[j], vcg[j]);
-      } else if (gw == 1) {
-        const zd yh = sYH[l * R + j];
-        pa += (G + kA) * (v[j].x * yh.x + v[j].y * yh.y);
-        v[j] = zscal(v[j], 0.5 * (G * G + 2.0 * G * kA + kA2) / (double)NG);
+  // ---- circulant products over all threads: K_H * rho, K_H * s, K_2 * gbar,
+  // K_1 * gbar, K_1 * y (Hartree potential and response, T_a gbar, P_a y) ----
+  {
+    const int P = blockDim.x / NG;
+    const int x = tid % NG, p = tid / NG;
+    if (p < P) {
+      double vh = 0.0, ks = 0.0;
+      zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0), py = mk(0.0, 0.0);
+      for (int t = p; t < NG; t += P) {
+        const int d = (x - t) & (NG - 1);
+        const double kh = sKH[d], k2 = sK2[d];
+        const zd k1 = sK1[d], gt = sG[t], yt = sY[t];
+        vh = fma(kh, sRho[t], vh);
+        ks = fma(kh, sS[t], ks);
+        c2.x = fma(k2, gt.x, c2.x);
+        c2.y = fma(k2, gt.y, c2.y);
+        c1 = zadd(c1, zmul(k1, gt));
+        py = zadd(py, zmul(k1, yt));
       }
-    });
-    fft.inv(v, T, l);
-#pragma unroll
-    for (int off = 1; off < L; off <<= 1) pa += __shfl_xor(pa, off);
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      const int x = L * m + l;
-      if (gw == 0) sVH[x] = v[m].x;
-      if (gw == 1) sTG[x] = v[m];
-      if (gw == 2) sKS[x] = v[m].x;
-    });
-    if (gw == 1 && l == 0) sRed[0] = pa / (double)NG;
+      double* q = reinterpret_cast<double*>(smem) + 8 * (size_t)tid;
+      q[0] = vh;
+      q[1] = ks;
+      q[2] = c2.x;
+      q[3] = c2.y;
+      q[4] = c1.x;
+      q[5] = c1.y;
+      q[6] = py.x;
+      q[7] = py.y;
+    }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- epilogue ----
+  double* stg = reinterpret_cast<double*>(smem) + 8 * (size_t)blockDim.x;  // abar terms
   if (tid < NG) {
+    double vh = 0.0, ks = 0.0;
+    zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0), py = mk(0.0, 0.0);
+    const int P = blockDim.x / NG;
+    for (int p = 0; p < P; ++p) {
+      const double* q = reinterpret_cast<const double*>(smem) + 8 * (size_t)(p * NG + tid);
+      vh += q[0];
+      ks += q[1];
+      c2.x += q[2];
+      c2.y += q[3];
+      c1.x += q[4];
+      c1.y += q[5];
+      py.x += q[6];
+      py.y += q[7];
+    }
+    vh *= a.occ;  // the density sum holds sum |y|^2
+    const double kA = kxv + fa;
+    // T_a gbar = K_2 * gbar + (k+a) K_1 * gbar + (k+a)^2/2 gbar
+    const zd tg = mk(c2.x + kA * c1.x + 0.5 * kA * kA * gv.x, c2.y + kA * c1.y + 0.5 * kA * kA * gv.y);
+    // dL/da term: Re conj(gbar) P_a y, P_a y = K_1 * y + (k+a) y
+    stg[tid] = gv.x * (py.x + kA * yv.x) + gv.y * (py.y + kA * yv.y);
     const double occr = out_occ ? a.occ : 0.0;
     if (egrad) {
       // dE/du = occ dv [2 (T y + (vion + vH) y) + (V_F + V'_F) y]
-      const zd h = zadd(sTG[tid], zscal(yv, vionv + sVH[tid]));
+      const zd h = zadd(tg, zscal(yv, vionv + vh));
       const zd c = zscal(zadd(zscal(h, 2.0), sACC[tid]), occr * a.dv);
-      a.cout[o] = c;
+      gstore(a.cout + o, c);
       const double w = a.dt / 6.0;  // gbar_4 = i (dt/6) c
-      a.gout[o] = mk(-w * c.y, w * c.x);
+      gstore(a.gout + o, mk(-w * c.y, w * c.x));
     } else {
-      zd vj = zadd(sTG[tid], zscal(gv, vionv + sVH[tid]));
-      vj = zadd(vj, zscal(yv, 2.0 * a.occ * (ib < a.nbocc ? 1.0 : 0.0) * sKS[tid]));
+      zd vj = zadd(tg, zscal(gv, vionv + vh));
+      vj = zadd(vj, zscal(yv, 2.0 * a.occ * (ib < a.nbocc ? 1.0 : 0.0) * ks));
       vj = zadd(vj, sACC[tid]);
-      const zd ubv = a.ub[o];
+      const zd ubv = gload(a.ub + o);
       zd uav, kb;
       if (a.mode == A_S4) {
         uav = zadd(ubv, vj);
         kb = zadd(zscal(ubv, a.dt / 3.0), zscal(vj, a.dt));
       } else if (a.mode == A_S3) {
-        uav = zadd(a.ua[o], vj);
+        uav = zadd(gload(a.ua + o), vj);
         kb = zadd(zscal(ubv, a.dt / 3.0), zscal(vj, 0.5 * a.dt));
       } else if (a.mode == A_S2) {
-        uav = zadd(a.ua[o], vj);
+        uav = zadd(gload(a.ua + o), vj);
         kb = zadd(zscal(ubv, a.dt / 6.0), zscal(vj, 0.5 * a.dt));
       } else {  // A_S1: cotangent of the state before the step; next step's gbar_4
-        uav = zadd(a.ua[o], vj);
+        uav = zadd(gload(a.ua + o), vj);
         kb = zscal(uav, a.dt / 6.0);
       }
-      a.ua[o] = uav;
-      a.gout[o] = mk(-kb.y, kb.x);  // i * kbar
+      gstore(a.ua + o, uav);
+      gstore(a.gout + o, mk(-kb.y, kb.x));  // i * kbar
     }
   }
-  if (!egrad && a.abar && tid == 0) a.abar[row] = sRed[0];
+  if (!egrad && a.abar) {
+    // fixed-order sum of the NG terms: chunks of 8, then the chunks
+    constexpr int NCH = (NG + 7) / 8;
+    lds_barrier();
+    if (tid < NCH) {
+      double s8 = 0.0;
+      for (int i = 0; i < 8 && 8 * tid + i < NG; ++i) s8 += stg[8 * tid + i];
+      sPart[tid] = s8;
+    }
+    lds_barrier();
+    if (tid == 0) {
+      double t = 0.0;
+      for (int c = 0; c < NCH; ++c) t += sPart[c];
+      gstore(a.abar + row, t);
+    }
+  }
 }
 
 // Current-squared objective L = dt sum_n J_n^2 (ubk_Je, src/Functions.f90
@@ -347,6 +384,13 @@ struct JgradArgs {
   int init;
   size_t tstride;
   int fstride;
+  // graph-replayable form: the state after step it - 1 is stage 1 of step
+  // it = *step_base - jstep in the tape (nothing to add for it = 0)
+  const int* step_base;
+  int jstep;
+  const zd* tape;
+  int tape_seg;
+  size_t blk;
 };
 
 template <int NG>
@@ -356,11 +400,19 @@ __global__ void __launch_bounds__(NG) jgrad_kernel(JgradArgs a) {
   const int x = threadIdx.x, row = blockIdx.x, tr = blockIdx.y;
   const int ik = row / a.nb, ib = row - ik * a.nb;
   const size_t o = (size_t)tr * a.tstride + (size_t)NG * row + x;
-  sy[x] = gload(a.y + o);
+  int step = a.step;
+  const zd* y = a.y;
+  if (a.step_base) {
+    const int it = *a.step_base - a.jstep;
+    if (it <= 0) return;  // uniform over the grid
+    step = it - 1;
+    y = a.tape + (size_t)(it % a.tape_seg) * 4 * a.blk;
+  }
+  sy[x] = gload(y + o);
   sk[x] = gload(reinterpret_cast<const zd*>(a.kc + 2 * NG) + x);
   __syncthreads();
-  const double A = gload(a.Aarr + (size_t)tr * a.fstride + a.step);
-  const double J = gload(a.J + (size_t)tr * a.fstride + a.step);
+  const double A = gload(a.Aarr + (size_t)tr * a.fstride + step);
+  const double J = gload(a.J + (size_t)tr * a.fstride + step);
   zd p = zscal(sy[x], gload(a.kx + ik) + A);
   for (int t = 0; t < NG; ++t) p = zadd(p, zmul(sk[(x - t) & (NG - 1)], sy[t]));
   const double occr = ib < a.nbocc ? a.occ : 0.0;

@@ -40,6 +40,8 @@ struct AdjState {
   double* res = nullptr;   // ntraj * (nt+1) * 7 reduced observables
   double* abar = nullptr;  // ntraj * seg * 4 * rows per-row field derivatives
   double* stsum = nullptr; // ntraj * nt * 4 stage field derivatives
+  // captured step blocks, keyed by the buffers they use
+  std::vector<std::pair<std::vector<const void*>, hipGraphExec_t>> graphs;
 };
 
 namespace {
@@ -61,7 +63,38 @@ void adj_free(AdjState& s) {
   dfree(s.res);
   dfree(s.abar);
   dfree(s.stsum);
+  for (auto& g : s.graphs) (void)hipGraphExecDestroy(g.second);
   s = AdjState();
+}
+
+// Steps per captured block of the taped forward and of the reverse sweep
+// (even: the reverse sweep's buffer rotation has period 2).
+constexpr int kAdjGraphSteps = 16;
+
+// Replays the captured block for key, capturing it with enqueue() first.
+template <class Enq>
+int adj_graph_launch(tdhf_ctx* c, AdjState& s, const std::vector<const void*>& key, Enq enqueue) {
+  hipGraphExec_t ex = nullptr;
+  for (auto& g : s.graphs)
+    if (g.first == key) ex = g.second;
+  if (!ex) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != 0) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, TDHF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return fail(c, TDHF_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
+    s.graphs.emplace_back(key, ex);
+  }
+  HIPCHK(c, hipGraphLaunch(ex, c->stream));
+  return 0;
 }
 
 size_t adj_lds_for(int ng, int nw) {
@@ -92,15 +125,6 @@ int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
     e1 = c->ev_pool[c->ev_used++];
     HIPCHK(c, hipEventRecord(e0, c->stream));
   }
-  const void* fn = nullptr;
-  switch (c->ng) {
-    case 16: fn = (const void*)tdhf::adjoint_kernel<16>; break;
-    case 32: fn = (const void*)tdhf::adjoint_kernel<32>; break;
-    case 64: fn = (const void*)tdhf::adjoint_kernel<64>; break;
-    case 128: fn = (const void*)tdhf::adjoint_kernel<128>; break;
-  }
-  if (lds > 64 * 1024)
-    HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   const dim3 grid(rows, ntraj), block(64 * nw);
   switch (c->ng) {
     case 16: hipLaunchKernelGGL(tdhf::adjoint_kernel<16>, grid, block, lds, c->stream, a); break;
@@ -113,8 +137,25 @@ int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
   return 0;
 }
 
+// Raises the LDS limit of the adjoint kernel (outside any stream capture).
+int adj_prepare(tdhf_ctx* c) {
+  int nw = choose_nw(c->ng, c->nk, c->nb);
+  while (nw > (c->ng + 63) / 64 && adj_lds_for(c->ng, nw) > 160 * 1024) --nw;
+  const size_t lds = adj_lds_for(c->ng, nw);
+  const void* fn = nullptr;
+  switch (c->ng) {
+    case 16: fn = (const void*)tdhf::adjoint_kernel<16>; break;
+    case 32: fn = (const void*)tdhf::adjoint_kernel<32>; break;
+    case 64: fn = (const void*)tdhf::adjoint_kernel<64>; break;
+    case 128: fn = (const void*)tdhf::adjoint_kernel<128>; break;
+  }
+  if (lds > 64 * 1024)
+    HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return 0;
+}
+
 int launch_jgrad(tdhf_ctx* c, const AdjState& s, int step, const zd* y, zd* ua, zd* gb,
-                 int init) {
+                 int init, int jstep = -1) {
   tdhf::JgradArgs j{};
   j.nb = c->nb;
   j.nbocc = c->nbocc;
@@ -133,6 +174,13 @@ int launch_jgrad(tdhf_ctx* c, const AdjState& s, int step, const zd* y, zd* ua, 
   j.init = init;
   j.tstride = nstate(c);
   j.fstride = s.nt;
+  if (jstep >= 0) {  // step and state from the device counter and the tape
+    j.step_base = c->d_step;
+    j.jstep = jstep;
+    j.tape = s.tape;
+    j.tape_seg = s.seg;
+    j.blk = (size_t)s.ntraj * nstate(c);
+  }
   const dim3 grid(c->nb * c->nk, s.ntraj);
   switch (c->ng) {
     case 16: hipLaunchKernelGGL(tdhf::jgrad_kernel<16>, grid, dim3(16), 0, c->stream, j); break;
@@ -155,8 +203,8 @@ tdhf::AdjArgs adj_base(const tdhf_ctx* c, const AdjState& s) {
   a.dt = c->dt;
   a.kx = c->d_kx;
   a.vion = c->d_vion;
-  a.vcgl = c->d_vcgl;
   a.vtl = c->d_vtl;
+  a.kc = c->d_kc;
   a.tw = c->d_tw;
   a.Aarr = s.dA;
   a.Aavearr = s.dAave;
@@ -166,9 +214,10 @@ tdhf::AdjArgs adj_base(const tdhf_ctx* c, const AdjState& s) {
 }
 
 // Forward steps [s0, s0+n) of every trajectory on the batch state u0 (s.st),
-// with the stage states written to tape (slot 0 = step s0) or, when
-// tape == nullptr, untaped.  Observables of step it go to slot it.
-int adj_forward_steps(tdhf_ctx* c, const AdjState& s, int s0, int n, zd* tape) {
+// with the stage states written to tape (step it in slot it % seg) or, when
+// tape == nullptr, untaped.  Observables of step it go to slot it.  Blocks of
+// kAdjGraphSteps steps are replayed from a captured graph.
+int adj_forward_steps(tdhf_ctx* c, AdjState& s, int s0, int n, zd* tape) {
   const int rows = c->nb * c->nk;
   const size_t ns = nstate(c), blk = (size_t)s.ntraj * ns;
   zd *u0 = s.st, *ks = s.st + blk, *sa = s.st + 2 * blk, *sb = s.st + 3 * blk;
@@ -180,61 +229,94 @@ int adj_forward_steps(tdhf_ctx* c, const AdjState& s, int s0, int n, zd* tape) {
   a.u0 = u0;
   a.ksum = ks;
   a.tape = tape;
-  a.tape_base = s0;
+  a.tape_seg = s.seg;
   a.nstate = blk;
   a.tstride = ns;
   a.fstride = s.nt;
   a.ostride = (size_t)(s.nt + 1) * c->obs_stride;
+  auto enq = [&](int m) -> int {
+    for (int j = 0; j < m; ++j) {
+      a.jstep = j;
+      a.mode = tdhf::M_S1; a.u = a.w = u0; a.unext = sa;
+      CHKRET(launch_apply(c, a, rows, s.ntraj));
+      a.mode = tdhf::M_S2; a.u = a.w = sa; a.unext = sb;
+      CHKRET(launch_apply(c, a, rows, s.ntraj));
+      a.mode = tdhf::M_S3; a.u = a.w = sb; a.unext = sa;
+      CHKRET(launch_apply(c, a, rows, s.ntraj));
+      a.mode = tdhf::M_S4; a.u = a.w = sa; a.unext = nullptr;
+      CHKRET(launch_apply(c, a, rows, s.ntraj));
+    }
+    hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, m);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+  };
   hipLaunchKernelGGL(tdhf::step_set_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, s0);
   HIPCHK(c, hipGetLastError());
-  for (int j = 0; j < n; ++j) {
-    a.jstep = j;
-    a.mode = tdhf::M_S1; a.u = a.w = u0; a.unext = sa;
-    CHKRET(launch_apply(c, a, rows, s.ntraj));
-    a.mode = tdhf::M_S2; a.u = a.w = sa; a.unext = sb;
-    CHKRET(launch_apply(c, a, rows, s.ntraj));
-    a.mode = tdhf::M_S3; a.u = a.w = sb; a.unext = sa;
-    CHKRET(launch_apply(c, a, rows, s.ntraj));
-    a.mode = tdhf::M_S4; a.u = a.w = sa; a.unext = nullptr;
-    CHKRET(launch_apply(c, a, rows, s.ntraj));
+  int done = 0;
+  if (!c->timing) {
+    const std::vector<const void*> key = {(const void*)1, tape, s.st, s.obs};
+    while (n - done >= kAdjGraphSteps) {
+      CHKRET(adj_graph_launch(c, s, key, [&] { return enq(kAdjGraphSteps); }));
+      done += kAdjGraphSteps;
+    }
   }
+  if (done < n) CHKRET(enq(n - done));
   return 0;
 }
 
 // Reverse sweep over steps [s0, s0+n) whose stage states are in s.tape
-// (slot 0 = step s0).  ub/ua/ga/gb rotate as described in tdhf_adjoint.h.
-// The per-row field derivatives of the segment go to s.abar (step it at
-// it - s0) and are summed per stage into s.stsum.
+// (step it in slot it % seg).  ub/ua/ga/gb rotate as described in
+// tdhf_adjoint.h.  The per-row field derivatives go to s.abar (same slots)
+// and are summed per stage into s.stsum.  Blocks of kAdjGraphSteps steps are
+// replayed from a captured graph (their buffer rotation returns to the start).
 int adj_backward_steps(tdhf_ctx* c, AdjState& s, int s0, int n, zd** ub, zd** ua, zd** ga,
                        zd** gb) {
   const size_t ns = nstate(c), blk = (size_t)s.ntraj * ns;
   const int rows = c->nb * c->nk;
   tdhf::AdjArgs a = adj_base(c, s);
   a.abar_tstride = (size_t)s.seg * 4 * rows;
-  for (int it = s0 + n - 1; it >= s0; --it) {
-    zd* tp = s.tape + (size_t)(it - s0) * 4 * blk;
-    a.step = it;
-    a.ub = *ub;
-    a.ua = *ua;
-    for (int st = 4; st >= 1; --st) {
-      a.mode = st;
-      a.y = tp + (size_t)(st - 1) * blk;
-      a.gb = *ga;
-      a.gout = *gb;
-      a.abar = s.abar + ((size_t)(it - s0) * 4 + (st - 1)) * rows;
-      CHKRET(launch_adj(c, a, s.ntraj));
-      std::swap(*ga, *gb);
+  a.step_base = c->d_step;
+  a.tape = s.tape;
+  a.tape_seg = s.seg;
+  a.blk = blk;
+  a.abar = s.abar;
+  // m steps from the counter down; rotates the four buffers in place
+  auto enq = [&](int m, zd** pub, zd** pua, zd** pga, zd** pgb) -> int {
+    for (int j = 0; j < m; ++j) {
+      a.jstep = j;
+      a.ub = *pub;
+      a.ua = *pua;
+      for (int st = 4; st >= 1; --st) {
+        a.mode = st;
+        a.gb = *pga;
+        a.gout = *pgb;
+        CHKRET(launch_adj(c, a, s.ntraj));
+        std::swap(*pga, *pgb);
+      }
+      // current-squared objective: J after step it-1 is evaluated on y1 of step it
+      if (s.objective == TDHF_OBJ_JSQ) CHKRET(launch_jgrad(c, s, 0, nullptr, *pua, *pga, 0, j));
+      std::swap(*pub, *pua);  // the accumulated cotangent becomes ubar of the step before
     }
-    // current-squared objective: J after step it-1 is evaluated on y1 of step it
-    if (s.objective == TDHF_OBJ_JSQ && it > 0) CHKRET(launch_jgrad(c, s, it - 1, tp, *ua, *ga, 0));
-    std::swap(*ub, *ua);  // the accumulated cotangent becomes ubar of the previous step
+    hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, -m);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+  };
+  hipLaunchKernelGGL(tdhf::step_set_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, s0 + n - 1);
+  HIPCHK(c, hipGetLastError());
+  const bool graphs = !c->timing && n >= kAdjGraphSteps;
+  const int r = graphs ? n % kAdjGraphSteps : n;
+  if (r > 0) CHKRET(enq(r, ub, ua, ga, gb));
+  for (int done = r; done < n; done += kAdjGraphSteps) {
+    const std::vector<const void*> key = {(const void*)2, *ub, *ua, *ga, *gb};
+    zd *qb = *ub, *qa = *ua, *qga = *ga, *qgb = *gb;  // rotated copies during capture
+    CHKRET(adj_graph_launch(c, s, key, [&] { return enq(kAdjGraphSteps, &qb, &qa, &qga, &qgb); }));
   }
   // stage sums of the segment, per trajectory: stsum[(t*nt + it)*4 + st]
   for (int t = 0; t < s.ntraj; ++t) {
     const long m = (long)n * 4;
     hipLaunchKernelGGL(tdhf::row_sum_kernel, dim3((unsigned)((m + 63) / 64)), dim3(64), 0,
-                       c->stream, s.abar + (size_t)t * s.seg * 4 * rows, rows, m,
-                       s.stsum + ((size_t)t * s.nt + s0) * 4);
+                       c->stream, s.abar + (size_t)t * s.seg * 4 * rows + (size_t)(s0 % s.seg) * 4 * rows,
+                       rows, m, s.stsum + ((size_t)t * s.nt + s0) * 4);
     HIPCHK(c, hipGetLastError());
   }
   return 0;
@@ -305,6 +387,7 @@ extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const d
   HIPCHK(c, hipMemcpyAsync(s.dAave, s.Aave.data(), s.Aave.size() * sizeof(double),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(s.st, u, bb, hipMemcpyHostToDevice, c->stream));
+  CHKRET(adj_prepare(c));
   if (s.full) {
     CHKRET(adj_forward_steps(c, s, 0, nt, s.tape));
   } else {

@@ -82,7 +82,7 @@ struct ApplyArgs {
   const TrajDesc* td;       // batched launch: trajectory table, or nullptr
   const int2* rowmap;       // batched launch: (trajectory, row) per workgroup
   zd* tape;                 // RK4 stage states y1..y4 per step (adjoint), or nullptr
-  int tape_base;            // first step held by the tape
+  int tape_seg;             // steps held by the tape: step it in slot it % tape_seg
   size_t nstate;            // tape stride of one stage block (all trajectories)
   // uniform batch (grid y = trajectory, same NK and tables): offsets of
   // trajectory t are t*tstride (states, zd), t*fstride (A, Aave), t*ostride
@@ -250,7 +250,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // taped forward: the stage states live in the tape slots of this step
   zd* tape0 = nullptr;
   if (a.tape && a.step_base) {
-    zd* tp = a.tape + (size_t)(step - a.tape_base) * 4 * a.nstate;
+    zd* tp = a.tape + (size_t)(step % a.tape_seg) * 4 * a.nstate;
     switch (a.mode) {
       case M_S1: tape0 = tp; unext = tp + a.nstate; break;
       case M_S2: u = w = tp + a.nstate; unext = tp + 2 * a.nstate; break;
