@@ -10,6 +10,7 @@ module tdhf_capi
   implicit none
 
   integer(c_int), parameter :: TDHF_OK = 0
+  integer(c_int), parameter :: TDHF_OBJ_ENERGY = 0, TDHF_OBJ_JSQ = 1
 
   type, bind(C) :: tdhf_params
     integer(c_int) :: ng, nk, nb, nbocc
@@ -99,6 +100,45 @@ module tdhf_capi
       import :: c_int, c_double_complex, c_ptr
       type(c_ptr), value :: ctx
       complex(c_double_complex), intent(inout) :: u(*)
+    end function
+
+    ! adjoint pair: gradient of Etot(u_N) - eref with respect to A(1:nt) and u0
+    integer(c_int) function tdhf_fwd(ctx, u, nt, A, eref, mem_budget, L, etot, nele, jtot) &
+         bind(C, name="tdhf_fwd")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(*)
+      real(c_double), value :: eref, mem_budget
+      real(c_double), intent(out) :: L, etot(*), nele(*), jtot(*)
+    end function
+
+    integer(c_int) function tdhf_bwd(ctx, dLdA, dLdu0) bind(C, name="tdhf_bwd")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dLdA(*)
+      complex(c_double_complex), intent(out) :: dLdu0(*)
+    end function
+
+    ! the same for ntraj pulses at once (u(NG,NB,NK,ntraj), A(nt,ntraj)) and
+    ! either objective (TDHF_OBJ_ENERGY, TDHF_OBJ_JSQ = dt*sum J_n^2)
+    integer(c_int) function tdhf_fwd_batch(ctx, ntraj, u, nt, A, objective, eref, mem_budget, &
+         L, etot, nele, jtot) bind(C, name="tdhf_fwd_batch")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ntraj, nt, objective
+      complex(c_double_complex), intent(inout) :: u(*)
+      real(c_double), intent(in) :: A(*), eref(*)
+      real(c_double), value :: mem_budget
+      real(c_double), intent(out) :: L(*), etot(*), nele(*), jtot(*)
+    end function
+
+    integer(c_int) function tdhf_bwd_batch(ctx, dLdA, dLdu0) bind(C, name="tdhf_bwd_batch")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dLdA(*)
+      complex(c_double_complex), intent(out) :: dLdu0(*)
     end function
   end interface
 
