@@ -153,7 +153,7 @@ size_t lds_for(int ng, int nw) {
 }
 
 int choose_nw(int ng, int nk, int nbocc) {
-  const int L = ng == 16 ? 4 : ng == 32 ? 4 : 8;
+  const int L = ng == 16 ? 4 : ng == 32 ? 4 : ng == 64 ? 8 : 16;
   const int gpw = 64 / L;
   const int items = nk * nbocc;
   int nw = (items + gpw - 1) / gpw;

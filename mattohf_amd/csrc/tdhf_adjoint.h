@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   }
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
-  constexpr int JNYQ = L / 2;  // register of lane 0 holding G index NG/2
+  constexpr int JNYQ = F::NYQ_REG;  // register of lane 0 holding G index NG/2
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;

@@ -175,7 +175,7 @@ struct Geo<64> {
 };
 template <>
 struct Geo<128> {
-  static constexpr int R = 16, L = 8;
+  static constexpr int R = 8, L = 16;  // GroupFFT<128> below
 };
 
 template <int NG>
@@ -185,6 +185,7 @@ struct GroupFFT {
   static constexpr int Q = R / L;
   static constexpr int PAD = L + 1;      // padded LDS row (bank spread)
   static constexpr int TSZ = R * PAD;    // complex elements of one group's tile
+  static constexpr int NYQ_REG = L / 2;  // lane 0 holds G index NG/2 in this register
   static_assert(R * L == NG && R % L == 0, "bad FFT geometry");
 
   zd fw[R];                    // exp(-2 pi i l g0 / NG), g0 = 0..R-1
@@ -293,6 +294,146 @@ struct GroupFFT {
   // G index held by lane l in register j of the G layout
   __host__ __device__ static constexpr int gindex(int l, int j) {
     return (l * Q + j / L) + R * (j % L);
+  }
+};
+
+// NG = 128 on 16 lanes x 8 points (the 8 x 16 split of the generic engine
+// would need 16 points per lane and does not fit the register file).
+// 128 = 8 * 4 * 4 with x = 16 m + 4 l2 + l1 and g = g0 + 8 h1 + 32 h2:
+//   exp(-2 pi i x g/128) = w8^(m g0) * w128^((4 l2 + l1) g0) * w4^(l2 h1)
+//                          * w16^(l1 h1) * w4^(l1 h2)
+// Forward: 8-point DFT over m in registers, twiddle w128^(l g0); transpose
+// so that lane (a, l1) = (lane/4, lane%4) holds g0 = 2a+s (s = 0, 1) and
+// l2 = 0..3; 4-point DFTs over l2, twiddle w16^(l1 h1); transpose within each
+// quad of lanes so that lane (a, c) holds h1 = c and l1 = 0..3; 4-point DFTs
+// over l1.  G layout: lane (a, c), register s*4 + h2 holds
+// g = (2a + s) + 8 c + 32 h2.  The inverse runs the same steps backwards with
+// conjugate twiddles.  Tile offsets are chosen so that neither transpose has
+// bank conflicts within a group (rows of 18 for the first, 44 a + 19 s +
+// 5 h1 + l1 for the second).
+template <>
+struct GroupFFT<128> {
+  static constexpr int NG = 128, R = 8, L = 16, Q = 1;
+  static constexpr int TSZ = 170;
+  static constexpr int NYQ_REG = 2;  // g = 64: lane 0, s = 0, h2 = 2
+
+  zd fw[8];   // w128^(l g0), g0 = 0..7 (lane l)
+  zd fq[4];   // w16^(l1 h1), h1 = 0..3 (l1 = lane % 4)
+
+  // tw[l*8 + g0] = w128^(l g0); tw[128 + l1*4 + h1] = w16^(l1 h1)
+  __device__ __forceinline__ void init(const zd* tw, int l) {
+    static_for<0, 8>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      fw[g0] = gload(tw + l * 8 + g0);
+    });
+    static_for<0, 4>([&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      fq[h] = gload(tw + 128 + (l & 3) * 4 + h);
+    });
+  }
+
+  __device__ static zd table_value(int idx) {
+    double sn, cs;
+    int e;
+    if (idx < 128) {
+      e = ((idx / 8) * (idx % 8)) % 128;
+    } else {
+      const int r = idx - 128;
+      e = r < 16 ? (((r / 4) * (r % 4)) % 16) * 8 : 0;
+    }
+    sincospi(2.0 * (double)e / 128.0, &sn, &cs);
+    return mk(cs, -sn);
+  }
+
+  static __device__ __forceinline__ int t1(int g0, int l) { return g0 * 18 + l; }
+  static __device__ __forceinline__ int t2(int a, int s, int h1, int l1) {
+    return 44 * a + 19 * s + 5 * h1 + l1;
+  }
+
+  __device__ __forceinline__ void fwd(zd (&v)[8], zd* T, int l) const {
+    const int a = l >> 2, l1 = l & 3;
+    zd y[8];
+    dft<8, -1, 1, 0, 8>(v, y);
+    static_for<1, 8>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      y[g0] = zmul(y[g0], fw[g0]);
+    });
+    static_for<0, 8>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      T[t1(g0, l)] = y[g0];
+    });
+    wave_sync();
+    zd z[8];
+    static_for<0, 8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value, s = i / 4, l2 = i % 4;
+      z[i] = T[t1(2 * a + s, 4 * l2 + l1)];
+    });
+    wave_sync();
+    static_for<0, 2>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[4];
+      dft<4, -1, 1, s * 4, 8>(z, t);
+      static_for<0, 4>([&](auto hc) {
+        constexpr int h1 = decltype(hc)::value;
+        T[t2(a, s, h1, l1)] = h1 == 0 ? t[0] : zmul(t[h1], fq[h1]);
+      });
+    });
+    wave_sync();
+    static_for<0, 8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value, s = i / 4, k1 = i % 4;
+      z[i] = T[t2(a, s, l1, k1)];  // lane (a, c = l1) gathers l1' = k1
+    });
+    wave_sync();
+    static_for<0, 2>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[4];
+      dft<4, -1, 1, s * 4, 8>(z, t);
+      static_for<0, 4>([&](auto hc) {
+        constexpr int h2 = decltype(hc)::value;
+        v[s * 4 + h2] = t[h2];
+      });
+    });
+  }
+
+  __device__ __forceinline__ void inv(zd (&v)[8], zd* T, int l) const {
+    const int a = l >> 2, c = l & 3;
+    static_for<0, 2>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[4];
+      dft<4, 1, 1, s * 4, 8>(v, t);  // over h2 -> l1
+      static_for<0, 4>([&](auto kc) {
+        constexpr int k1 = decltype(kc)::value;
+        T[t2(a, s, c, k1)] = k1 == 0 ? t[0] : zmulc(t[k1], fq[k1]);  // w16^(-c l1)
+      });
+    });
+    wave_sync();
+    zd z[8];
+    static_for<0, 8>([&](auto ic) {
+      constexpr int i = decltype(ic)::value, s = i / 4, h1 = i % 4;
+      z[i] = T[t2(a, s, h1, c)];  // lane (a, l1 = c) gathers h1
+    });
+    wave_sync();
+    static_for<0, 2>([&](auto sc) {
+      constexpr int s = decltype(sc)::value;
+      zd t[4];
+      dft<4, 1, 1, s * 4, 8>(z, t);  // over h1 -> l2
+      static_for<0, 4>([&](auto kc) {
+        constexpr int l2 = decltype(kc)::value;
+        T[t1(2 * a + s, 4 * l2 + c)] = t[l2];
+      });
+    });
+    wave_sync();
+    zd y[8];
+    static_for<0, 8>([&](auto gc) {
+      constexpr int g0 = decltype(gc)::value;
+      y[g0] = g0 == 0 ? T[t1(0, l)] : zmulc(T[t1(g0, l)], fw[g0]);
+    });
+    wave_sync();
+    dft<8, 1, 1, 0, 8>(y, v);
+  }
+
+  __host__ __device__ static constexpr int gindex(int l, int j) {
+    return (2 * (l >> 2) + j / 4) + 8 * (l & 3) + 32 * (j % 4);
   }
 };
 
