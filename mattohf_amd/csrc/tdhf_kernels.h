@@ -143,18 +143,26 @@ __device__ __forceinline__ void set_wave_priority(int p) {
 }
 
 // Scratch doubles after the fixed LDS arrays of apply_kernel: the
-// group_reduce partials, max(threads, 2 NG).
+// group_reduce partials of its 3 NG values, max(threads, 3 NG).
 __host__ __device__ constexpr size_t apply_part_doubles(int nw, int NG) {
-  return (size_t)64 * nw > (size_t)2 * NG ? (size_t)64 * nw : (size_t)2 * NG;
+  return (size_t)64 * nw > (size_t)3 * NG ? (size_t)64 * nw : (size_t)3 * NG;
 }
 
-// LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' tiles
-// (transposes during the pair loop, reduction slots and circulant-product
-// partials after it), then w, V_F w, K_1 (complex NG each), rho, K_H, K_2 (NG
-// doubles each), a small scratch and the reduction scratch.
+// Per-group LDS slot of apply_kernel (complex elements): the transpose tile
+// during the pair loop, then the group's V_F w (2 NG doubles) and density
+// (NG doubles) for the single reduction round.
+template <int NG>
+__host__ __device__ constexpr int apply_slot() {
+  return GroupFFT<NG>::TSZ >= (3 * NG + 1) / 2 ? GroupFFT<NG>::TSZ : (3 * NG + 1) / 2;
+}
+
+// LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' slots
+// (also the circulant-product partials and observables staging after the
+// pair loop), then w, V_F w (followed by rho), K_1 (complex NG each), rho,
+// K_H, K_2 (NG doubles each), a small scratch and the reduction scratch.
 template <int NG>
 __host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
-  return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
+  return (size_t)nw * (64 / GroupFFT<NG>::L) * apply_slot<NG>() * sizeof(zd) +
          3 * NG * sizeof(zd) + 3 * NG * sizeof(double) + 16 * sizeof(double) +
          apply_part_doubles(nw, NG) * sizeof(double);
 }
@@ -163,13 +171,13 @@ template <int NG>
 __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
-  static_assert(2 * NG <= 2 * F::TSZ, "reduction slot must fit in a tile");
-  // after the reductions the tile area holds 5 partials per thread and the
-  // observables staging (7 NG doubles); checked at the smallest launch
-  // (one wave, or NG/64 waves)
+  constexpr int SSZ = apply_slot<NG>();
+  // after the reduction the slot area holds 5 partials per thread and the
+  // observables staging (7 NG doubles at 5 * blockDim); checked at the
+  // smallest launch (one wave, or NG/64 waves)
   static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
-                    2 * F::TSZ * GPW * (NG > 64 ? NG / 64 : 1),
-                "tile area too small for the epilogue scratch");
+                    2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
+                "slot area too small for the epilogue scratch");
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -217,12 +225,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   }
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
-  zd* T = smem + grp * F::TSZ;
-  zd* sW = smem + ngrp * F::TSZ;
+  zd* T = smem + grp * SSZ;
+  zd* sW = smem + ngrp * SSZ;
   zd* sVF = sW + NG;
-  zd* sK1 = sVF + NG;
-  double* sRho = reinterpret_cast<double*>(sK1 + NG);
-  double* sKH = sRho + NG;
+  double* sRho = reinterpret_cast<double*>(sVF + NG);  // follows V_F w (one reduction)
+  zd* sK1 = reinterpret_cast<zd*>(sRho + NG);
+  double* sKH = reinterpret_cast<double*>(sK1 + NG);
   double* sK2 = sKH + NG;
   double* sRed = sK2 + NG;
   double* sPart = sRed + 16;  // apply_part_doubles(nw, NG)
@@ -271,9 +279,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     if (obs) obs += (size_t)tr * a.ostride;
   }
 
-  // ---- prologue: w of this row into LDS ----
+  // ---- prologue: w of this row and the kinetic kernels into LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
-  if (tid < NG) sW[tid] = gload(w + o);
+  if (tid < NG) {
+    sW[tid] = gload(w + o);
+    sK2[tid] = gload(kc + NG + tid);
+    sK1[tid] = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
+  }
   const double kxv = gload(kx + ik);
   F fft;
   fft.init(a.tw, l);
@@ -307,6 +319,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // until it catches up.
   const int wave_items = (nitem - wave * GPW + ngrp - 1) / ngrp;  // of its first group
   lds_barrier();  // sW visible
+
   int it = 0;
   for (int item = grp; item < nitem; item += ngrp, ++it) {
     set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
@@ -346,14 +359,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   }
   set_wave_priority(0);
 
-  // epilogue operands: loaded now, used after the reductions (whose barriers
+  // epilogue operands: loaded now, used after the reduction (whose barriers
   // order LDS only, so these loads stay in flight meanwhile)
-  zd u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0), k1v = mk(0.0, 0.0);
-  double vionv = 0.0, khv = 0.0, k2v = 0.0;
+  zd u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0);
+  double vionv = 0.0, khv = 0.0;
   if (tid < NG) {
     khv = gload(kc + tid);
-    k2v = gload(kc + NG + tid);
-    k1v = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
     vionv = gload(vion + tid);
     if (a.mode >= M_S1 && a.mode <= M_S4) {
       u0v = gload(u0 + o);
@@ -361,50 +372,38 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
-  // ---- reduction over all groups through LDS (fixed order) ----
-  // round 1: V_F w; each group parks its partial in its own tile
+  // ---- one reduction over all groups through LDS (fixed order): each group
+  // parks V_F w (2 NG doubles) and its density sum (NG doubles) in its slot;
+  // the sums land in sVF and, right after it, sRho (sum |u|^2, without occ) ----
   double* slotd = reinterpret_cast<double*>(T);
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     slotd[2 * (L * m + l)] = acc[m].x;
     slotd[2 * (L * m + l) + 1] = acc[m].y;
+    slotd[2 * NG + L * m + l] = den[m];
   });
   lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 3 * NG, sPart,
                reinterpret_cast<double*>(sVF), 1.0);
-  lds_barrier();
-  // round 2: density
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    slotd[L * m + l] = den[m];
-  });
-  lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, NG, sPart, sRho, occ);
-  if (tid < NG) {
-    sKH[tid] = khv;
-    sK2[tid] = k2v;
-    sK1[tid] = k1v;
-  }
+  if (tid < NG) sKH[tid] = khv;
   lds_barrier();
 
   // ---- circulant products over all threads: K_H * rho, K_2 * w, K_1 * w ----
   // thread (p, x) sums the terms y = p, p + P, ... into partials parked in the
-  // tile area (free after the reductions), then thread x adds the P partials
+  // slot area (free after the reduction), then thread x adds the P partials
   // in order.
   {
     const int P = blockDim.x / NG;
     const int x = tid % NG, p = tid / NG;
-    double* cp = reinterpret_cast<double*>(smem);
     if (p < P) {
       double vh = 0.0;
       zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
       for (int y = p; y < NG; y += P) {
         const int d = (x - y) & (NG - 1);
-        const double r = sRho[y];
         const zd wy = sW[y];
         const double k2 = sK2[d];
         const zd k1 = sK1[d];
-        vh = fma(sKH[d], r, vh);
+        vh = fma(sKH[d], sRho[y], vh);
         c2.x = fma(k2, wy.x, c2.x);
         c2.y = fma(k2, wy.y, c2.y);
         c1.x = fma(k1.x, wy.x, c1.x);
@@ -412,7 +411,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         c1.y = fma(k1.x, wy.y, c1.y);
         c1.y = fma(k1.y, wy.x, c1.y);
       }
-      double* q = cp + 5 * (size_t)tid;
+      double* q = reinterpret_cast<double*>(smem) + 5 * (size_t)tid;
       q[0] = vh;
       q[1] = c2.x;
       q[2] = c2.y;
@@ -438,6 +437,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       c1.x += q[3];
       c1.y += q[4];
     }
+    vh *= occ;  // the density sum holds sum |u|^2
     const double kA = kxv + Akin;
     const zd vf = sVF[tid];
     // T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w
@@ -468,7 +468,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       const double w2 = zabs2(wv);
       const double wc1 = wv.x * c1.x + wv.y * c1.y;  // Re conj(w) (K_1 * w)
       const double wc2 = wv.x * c2.x + wv.y * c2.y;
-      const double rho = sRho[tid];
+      const double rho = occ * sRho[tid];
       double* st = reinterpret_cast<double*>(smem) + 5 * (size_t)blockDim.x;
       st[0 * NG + tid] = wc2 + kAo * wc1 + 0.5 * kAo * kAo * w2;
       st[1 * NG + tid] = wc1 + kAo * w2;
