@@ -150,10 +150,14 @@ __host__ __device__ constexpr size_t apply_part_doubles(int nw, int NG) {
 
 // Per-group LDS slot of apply_kernel (complex elements): the transpose tile
 // during the pair loop, then the group's V_F w (2 NG doubles) and density
-// (NG doubles) for the single reduction round.
+// (NG doubles) for the single reduction round.  Rounded up to 8 mod 16
+// elements, so that consecutive groups start half a bank cycle (32 of the 64
+// four-byte banks) apart and the transposes of two groups served in one LDS
+// pass do not collide.
 template <int NG>
 __host__ __device__ constexpr int apply_slot() {
-  return GroupFFT<NG>::TSZ >= (3 * NG + 1) / 2 ? GroupFFT<NG>::TSZ : (3 * NG + 1) / 2;
+  constexpr int need = GroupFFT<NG>::TSZ >= (3 * NG + 1) / 2 ? GroupFFT<NG>::TSZ : (3 * NG + 1) / 2;
+  return need + ((8 - need % 16) + 16) % 16;
 }
 
 // LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' slots
