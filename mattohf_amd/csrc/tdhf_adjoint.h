@@ -60,6 +60,16 @@ struct AdjArgs {
   size_t abar_tstride;
 };
 
+// Per-group LDS slot of adjoint_kernel (complex elements): the transpose
+// tile, then the group's exchange sum (2 NG doubles), density and s (NG
+// doubles each) for the single reduction round; 8 mod 16 elements as in
+// apply_slot.
+template <int NG>
+__host__ __device__ constexpr int adjoint_slot() {
+  constexpr int need = GroupFFT<NG>::TSZ >= 2 * NG ? GroupFFT<NG>::TSZ : 2 * NG;
+  return need + ((8 - need % 16) + 16) % 16;
+}
+
 // LDS of adjoint_kernel<NG> with nw waves (bytes): the groups' tiles (pair
 // transposes, then reduction slots, circulant-product partials and the field
 // derivative staging), y, gbar, the exchange sum and K_1 (complex NG each),
@@ -67,9 +77,9 @@ struct AdjArgs {
 // partials.
 template <int NG>
 __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
-  return (size_t)nw * (64 / GroupFFT<NG>::L) * GroupFFT<NG>::TSZ * sizeof(zd) +
+  return (size_t)nw * (64 / GroupFFT<NG>::L) * adjoint_slot<NG>() * sizeof(zd) +
          4 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
-         ((size_t)64 * nw > 2 * NG ? (size_t)64 * nw : 2 * NG) * sizeof(double);
+         ((size_t)64 * nw > 4 * NG ? (size_t)64 * nw : 4 * NG) * sizeof(double);
 }
 
 template <int NG>
@@ -112,19 +122,20 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   const bool egrad = a.mode == A_EGRAD;
   const zd* gbar = egrad ? a.y : a.gb;
 
-  zd* T = smem + grp * F::TSZ;
-  zd* sY = smem + ngrp * F::TSZ;   // y_ki
+  constexpr int SSZ = adjoint_slot<NG>();
+  zd* T = smem + grp * SSZ;
+  zd* sY = smem + ngrp * SSZ;      // y_ki
   zd* sG = sY + NG;                // gbar_ki
-  zd* sACC = sG + NG;              // exchange part of J^T gbar
-  zd* sK1 = sACC + NG;
-  double* sRho = reinterpret_cast<double*>(sK1 + NG);
+  zd* sK1 = sG + NG;
+  zd* sACC = sK1 + NG;             // exchange part of J^T gbar, then rho and s
+  double* sRho = reinterpret_cast<double*>(sACC + NG);
   double* sS = sRho + NG;
   double* sKH = sS + NG;
   double* sK2 = sKH + NG;
   double* sRed = sK2 + NG;
-  double* sPart = sRed + 16;  // max(blockDim.x, 2 NG) doubles for group_reduce
-  static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * F::TSZ * GPW * (NG > 64 ? NG / 64 : 1),
-                "tile area too small for the epilogue scratch");
+  double* sPart = sRed + 16;  // max(blockDim.x, 4 NG) doubles for group_reduce
+  static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
+                "slot area too small for the epilogue scratch");
 
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   zd yv = mk(0.0, 0.0), gv = mk(0.0, 0.0);
@@ -239,24 +250,19 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
     }
   }
 
-  // ---- reductions over the groups (fixed order) ----
+  // ---- one reduction over the groups (fixed order): exchange sum, density
+  // and s land in sACC and, right after it, sRho and sS ----
   double* slotd = reinterpret_cast<double*>(T);
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     slotd[2 * (L * m + l)] = acc[m].x;
     slotd[2 * (L * m + l) + 1] = acc[m].y;
+    slotd[2 * NG + L * m + l] = den[m];
+    slotd[3 * NG + L * m + l] = sv[m];
   });
   lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart,
+  group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 4 * NG, sPart,
                reinterpret_cast<double*>(sACC), 1.0);
-  lds_barrier();
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    slotd[L * m + l] = den[m];
-    slotd[NG + L * m + l] = sv[m];
-  });
-  lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * F::TSZ, ngrp, 2 * NG, sPart, sRho, 1.0);
   lds_barrier();
 
   // ---- circulant products over all threads: K_H * rho, K_H * s, K_2 * gbar,
