@@ -74,6 +74,10 @@ def lib():
             "orc_objective_energy": (d, [vp, vp, d, i, vp, d]),
             "orc_adjoint_jsq": (d, [vp, vp, d, i, vp, vp, vp]),
             "orc_objective_jsq": (d, [vp, vp, d, i, vp]),
+            "orc_gs_run_diag": (None, [vp, vp, i, i, vp, vp, vp, vp]),
+            "orc_make_dos": (None, [i, i, i, vp, i, vp, vp, vp]),
+            "orc_fourier": (None, [i, d, vp, vp, d, vp, vp, i, d, i, d, i, vp]),
+            "orc_te_step": (None, [vp, vp, d, d, i]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -286,6 +290,44 @@ class Oracle:
         err = np.empty((self.sys.nb, self.sys.nk), order="F")
         lib().orc_gs_run(self.h, _p(u), int(nscf), int(ncg), _p(eh), _p(eps), _p(err))
         return u, eh, eps, err
+
+
+    def GS_calculation_diag(self, u, nscf, ncg):
+        """GS_calculation with the per-iteration columns of _GSinfo.out:
+        (u, etot_hist, epsbk, epserr, diag(nscf, 5)) with diag = (mean |d dns|,
+        mean err, max err, fundamental gap, optical gap)."""
+        u = self._u(u).copy(order="F")
+        eh = np.empty(nscf + 1)
+        eps = np.empty((self.sys.nb, self.sys.nk), order="F")
+        err = np.empty((self.sys.nb, self.sys.nk), order="F")
+        dg = np.empty((max(nscf, 1), 5))
+        lib().orc_gs_run_diag(self.h, _p(u), int(nscf), int(ncg), _p(eh), _p(eps), _p(err), _p(dg))
+        return u, eh, eps, err, dg[:nscf]
+
+    def TE_step(self, u, dt, Aave, variant=0):
+        """RT_TE_ubk (0), RT_TEPC_ubk (1), RT_TEPC2_ubk (2): one step."""
+        u = self._u(u).copy(order="F")
+        lib().orc_te_step(self.h, _p(u), float(dt), float(Aave), int(variant))
+        return u
+
+
+def make_dos(epsbk, nbocc, ndos=10000):
+    """Make_DoS on the GS_calculation energy grid: (epsdos, dos, dosocc)."""
+    e = np.asfortranarray(np.asarray(epsbk, dtype=np.float64))
+    nb, nk = e.shape
+    x, d, q = np.empty(ndos), np.empty(ndos), np.empty(ndos)
+    lib().orc_make_dos(nb, nk, int(nbocc), _p(e), int(ndos), _p(x), _p(d), _p(q))
+    return x, d, q
+
+
+def fourier(time, J, j0, E, A, dt, nomega, omegam, window=0, tpulse=0.0, nenvelope=2):
+    """FourierTr (window 0) / WindowFourierTr (window 1): (nomega, 7) rows
+    omega, Re/Im JF, Re/Im EF, Re/Im AF."""
+    t, J, E, A = (np.ascontiguousarray(v, dtype=np.float64) for v in (time, J, E, A))
+    out = np.empty((int(nomega), 7))
+    lib().orc_fourier(t.size, float(dt), _p(t), _p(J), float(j0), _p(E), _p(A), int(nomega),
+                      float(omegam), int(window), float(tpulse), int(nenvelope), _p(out))
+    return out
 
 
 def heev(a):

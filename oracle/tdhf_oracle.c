@@ -22,6 +22,7 @@
  */
 #include "tdhf_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -788,6 +789,183 @@ void orc_gs_run(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_h
     if (etot_hist) etot_hist[iter] = orc_etot(c, ubk, 0.0);
   }
   if (epsbk || epserr) orc_epsbk(c, ubk, epsbk, epserr);
+}
+
+/* GS_calculation with its per-iteration diagnostics (src/GS_calculation.f90
+ * :28-57): diag[5*(iter-1) + ...] = mean |d dnsscf| (dnsscf mixed with 0.5,
+ * :27,44), mean and max epserrorbk, fundamental gap min eps(NBocc+1:,:) -
+ * max eps(:NBocc,:), optical gap min_k eps(NBocc+1,k) - eps(NBocc,k).  With
+ * NB == NBocc the gaps are DBL_MAX (Fortran minval of an empty set is
+ * huge(0d0); the optical gap would read past the band range there). */
+void orc_gs_run_diag(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_hist,
+                     double* epsbk, double* epserr, double* diag) {
+  const int NG = c->s.ng, NB = c->s.nb, NK = c->s.nk, NBocc = c->s.nbocc;
+  double* dmix = malloc(sizeof(double) * NG);
+  double* dn = malloc(sizeof(double) * NG);
+  double* eps = malloc(sizeof(double) * NB * NK);
+  double* err = malloc(sizeof(double) * NB * NK);
+  orc_dns(c, ubk, dmix);
+  if (etot_hist) etot_hist[0] = orc_etot(c, ubk, 0.0);
+  for (int iter = 1; iter <= nscf; ++iter) {
+    orc_gram_schmidt(&c->s, ubk);
+    orc_ks_krylov(c, ubk, ncg);
+    orc_gram_schmidt(&c->s, ubk);
+    orc_subspace_diag(c, ubk);
+    if (etot_hist) etot_hist[iter] = orc_etot(c, ubk, 0.0);
+    orc_epsbk(c, ubk, eps, err);
+    orc_dns(c, ubk, dn);
+    double dd = 0.0;
+    for (int ig = 0; ig < NG; ++ig) {
+      const double nx = 0.5 * dn[ig] + 0.5 * dmix[ig];
+      dd += fabs(nx - dmix[ig]);
+      dmix[ig] = nx;
+    }
+    double se = 0.0, me = -DBL_MAX;
+    for (int i = 0; i < NB * NK; ++i) {
+      se += err[i];
+      if (err[i] > me) me = err[i];
+    }
+    double vtop = -DBL_MAX, cbot = DBL_MAX, opt = DBL_MAX;
+    for (int ik = 0; ik < NK; ++ik)
+      for (int ib = 0; ib < NB; ++ib) {
+        const double e = eps[ib + NB * ik];
+        if (ib < NBocc && e > vtop) vtop = e;
+        if (ib >= NBocc && e < cbot) cbot = e;
+      }
+    if (NB > NBocc)
+      for (int ik = 0; ik < NK; ++ik) {
+        const double g = eps[NBocc + NB * ik] - eps[NBocc - 1 + NB * ik];
+        if (g < opt) opt = g;
+      }
+    double* dg = diag + (size_t)5 * (iter - 1);
+    dg[0] = dd / (double)NG;
+    dg[1] = se / (double)(NB * NK);
+    dg[2] = me;
+    dg[3] = NB > NBocc ? cbot - vtop : DBL_MAX;
+    dg[4] = opt;
+  }
+  if (epsbk) memcpy(epsbk, eps, sizeof(double) * NB * NK);
+  if (epserr) memcpy(epserr, err, sizeof(double) * NB * NK);
+  if (nscf == 0 && (epsbk || epserr)) orc_epsbk(c, ubk, epsbk, epserr);
+  free(dmix); free(dn); free(eps); free(err);
+}
+
+/* Make_DoS with the energy grid of GS_calculation (src/GS_calculation.f90
+ * :93-97, :239-265): epsdos[i] = emin + (emax-emin) i/ndos (i = 0..ndos-1),
+ * emin/emax = min/max eps -/+ 10 % of the range; Gaussians of width
+ * 5 (epsdos[1]-epsdos[0]) weighted by max occ of the k-point (dos) and by the
+ * band occupation (dosocc, occ = 2/NK for ib < NBocc). */
+void orc_make_dos(int nb, int nk, int nbocc, const double* epsbk, int ndos, double* epsdos,
+                  double* dos, double* dosocc) {
+  double emin = DBL_MAX, emax = -DBL_MAX;
+  for (int i = 0; i < nb * nk; ++i) {
+    if (epsbk[i] < emin) emin = epsbk[i];
+    if (epsbk[i] > emax) emax = epsbk[i];
+  }
+  const double lo = emin - 0.1 * (emax - emin), hi = emax + 0.1 * (emax - emin);
+  for (int i = 0; i < ndos; ++i) epsdos[i] = lo + (hi - lo) * (double)i / (double)ndos;
+  const double ew = 5.0 * (epsdos[1] - epsdos[0]);
+  const double occ = 2.0 / (double)nk, occmax = nbocc > 0 ? occ : 0.0;
+  const double norm = ew * sqrt(M_PI);
+  for (int i = 0; i < ndos; ++i) {
+    double d = 0.0, dq = 0.0;
+    for (int ik = 0; ik < nk; ++ik)
+      for (int ib = 0; ib < nb; ++ib) {
+        const double x = epsbk[ib + nb * ik] - epsdos[i];
+        const double v = exp(-x * x / (ew * ew));
+        d += occmax * v;
+        dq += (ib < nbocc ? occ : 0.0) * v;
+      }
+    dos[i] = d / norm;
+    dosocc[i] = dq / norm;
+  }
+}
+
+/* FourierTr (window 0) and WindowFourierTr (window 1), src/RT_calculation.f90
+ * :246-351: omega_i = i/nomega * omegam (i = 1..nomega); filter 0:
+ * 1 - 3 (t/t_N)^2 + 2 (t/t_N)^3, filter 1: sin(pi t/tpulse)^nenvelope for
+ * the first int(min(tpulse/dt, nt)) steps, else 0; JF = sum e^{i w t}
+ * filter (J - j0) dt, EF and AF likewise without j0.  out[7 i + ...] = omega,
+ * Re JF, Im JF, Re EF, Im EF, Re AF, Im AF. */
+void orc_fourier(int nt, double dt, const double* time, const double* J, double j0,
+                 const double* E, const double* A, int nomega, double omegam, int window,
+                 double tpulse, int nenvelope, double* out) {
+  double* f = malloc(sizeof(double) * nt);
+  const double tn = time[nt - 1];
+  int ntp = 0;
+  if (window == 1) {
+    const double q = tpulse / dt < (double)nt ? tpulse / dt : (double)nt;
+    ntp = (int)q;
+  }
+  for (int it = 0; it < nt; ++it) {
+    if (window == 0) {
+      const double r = time[it] / tn;
+      f[it] = 1.0 - 3.0 * r * r + 2.0 * r * r * r;
+    } else {
+      f[it] = it < ntp ? pow(sin(M_PI * time[it] / tpulse), (double)nenvelope) : 0.0;
+    }
+  }
+  for (int iw = 0; iw < nomega; ++iw) {
+    const double w = (double)(iw + 1) / (double)nomega * omegam;
+    double jr = 0, ji = 0, er = 0, ei = 0, ar = 0, ai = 0;
+    for (int it = 0; it < nt; ++it) {
+      const double cs = cos(w * time[it]), sn = sin(w * time[it]);
+      const double jt = f[it] * (J[it] - j0), et = f[it] * E[it], at = f[it] * A[it];
+      jr += cs * jt; ji += sn * jt;
+      er += cs * et; ei += sn * et;
+      ar += cs * at; ai += sn * at;
+    }
+    double* o = out + (size_t)7 * iw;
+    o[0] = w;
+    o[1] = jr * dt; o[2] = ji * dt;
+    o[3] = er * dt; o[4] = ei * dt;
+    o[5] = ar * dt; o[6] = ai * dt;
+  }
+  free(f);
+}
+
+/* Taylor propagators, src/RT_calculation.f90:71-151 (Norder = 4, H frozen):
+ * variant 0 RT_TE_ubk, 1 RT_TEPC_ubk, 2 RT_TEPC2_ubk. */
+static void te_series(const orc_ctx* c, const zc* hu, const zc* hu2, const zc* w0, double f,
+                      double Aave, zc* u) {
+  /* u += sum_{k=1..4} (-i f dt_eff ...): w_k = -i f (H w_{k-1})/k with H =
+   * H[hu] (or the mean of H[hu] and H[hu2]) */
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  zc* w = malloc(sizeof(zc) * n);
+  zc* h = malloc(sizeof(zc) * n);
+  zc* h2 = malloc(sizeof(zc) * n);
+  memcpy(w, w0, sizeof(zc) * n);
+  for (int k = 1; k <= 4; ++k) {
+    orc_hpsi(c, (const double*)hu, (const double*)w, Aave, (double*)h);
+    if (hu2) {
+      orc_hpsi(c, (const double*)hu2, (const double*)w, Aave, (double*)h2);
+      for (size_t i = 0; i < n; ++i) h[i] = 0.5 * (h[i] + h2[i]);
+    }
+    for (size_t i = 0; i < n; ++i) w[i] = -I * f * h[i] / (double)k;
+    for (size_t i = 0; i < n; ++i) u[i] += w[i];
+  }
+  free(w); free(h); free(h2);
+}
+
+void orc_te_step(const orc_ctx* c, double* ubk_, double dt, double Aave, int variant) {
+  zc* ubk = (zc*)ubk_;
+  const size_t n = (size_t)c->s.ng * c->s.nb * c->s.nk;
+  zc* u0 = malloc(sizeof(zc) * n);
+  zc* u1 = malloc(sizeof(zc) * n);
+  memcpy(u0, ubk, sizeof(zc) * n);
+  if (variant == 0) {
+    te_series(c, u0, NULL, u0, dt, Aave, ubk);
+  } else {
+    memcpy(u1, u0, sizeof(zc) * n);
+    te_series(c, u0, NULL, u0, dt, Aave, u1);  /* predictor */
+    if (variant == 1) {
+      te_series(c, u0, u1, u0, dt, Aave, ubk);
+    } else {
+      for (size_t i = 0; i < n; ++i) u1[i] = 0.5 * (u0[i] + u1[i]);
+      te_series(c, u1, NULL, u0, dt, Aave, ubk);
+    }
+  }
+  free(u0); free(u1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -71,6 +71,15 @@ void   orc_subspace_diag(const orc_ctx* c, double* ubk);
 void   orc_gs_run(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_hist,
                   double* epsbk, double* epserr);
 
+void   orc_gs_run_diag(const orc_ctx* c, double* ubk, int nscf, int ncg, double* etot_hist,
+                       double* epsbk, double* epserr, double* diag);
+void   orc_make_dos(int nb, int nk, int nbocc, const double* epsbk, int ndos, double* epsdos,
+                    double* dos, double* dosocc);
+void   orc_fourier(int nt, double dt, const double* time, const double* J, double j0,
+                   const double* E, const double* A, int nomega, double omegam, int window,
+                   double tpulse, int nenvelope, double* out);
+void   orc_te_step(const orc_ctx* c, double* ubk, double dt, double Aave, int variant);
+
 double orc_adjoint_energy(const orc_ctx* c, const double* u0, double dt, int nt, const double* A,
                           double eref, double* dLdA, double* dLdu0);
 double orc_objective_energy(const orc_ctx* c, const double* u0, double dt, int nt,

@@ -30,6 +30,14 @@ function is therefore checked against something independent of its code:
   orc_adjoint_*      central finite differences of the oracle's own forward
                      objective (energy at the end and dt*sum J^2), in the
                      field and in the initial orbitals
+  orc_gs_run_diag    stepwise recomposition from one-iteration GS runs,
+                     ubk_dns and ubk_epsbk
+  orc_make_dos       Gaussian normalisation integrals, single-level peak,
+                     energy-grid ends
+  orc_fourier        numpy FFT on the transform's own frequency grid, both
+                     filters
+  orc_te_step        4th-order Taylor polynomial of dense H[u] matrices built
+                     column by column (TE, TEPC, TEPC2)
 """
 import math
 
@@ -397,3 +405,107 @@ def test_oracle_adjoint_finite_differences(kind):
     du = rng.standard_normal(S.shape) + 1j * rng.standard_normal(S.shape)
     fd = (f(u0 + h * du, A) - f(u0 - h * du, A)) / (2 * h)
     assert abs(fd - np.real(np.vdot(gu, du))) <= 1e-6 * abs(fd) + 1e-11
+
+
+# ---- widening: GS diagnostics, Make_DoS, spectra, Taylor propagators ----
+
+def test_gs_diag_matches_stepwise_recomposition():
+    """orc_gs_run_diag's columns against the same quantities rebuilt from
+    one-iteration GS runs, ubk_dns and ubk_epsbk (mixing 0.5 in numpy)."""
+    S = po.System(16, 2, 3, 2, 5.0)
+    vion = S.ion_potential([2.0], [1.0], [0.0])
+    O = po.Oracle(S, vion)
+    u0 = S.gram_schmidt(S.init_wf())
+    u, eh, eps, err, dg = O.GS_calculation_diag(u0, 4, 3)
+    uu = u0.copy()
+    dmix = O.ubk_dns(uu)
+    for it in range(4):
+        uu = O.GS_calculation(uu, 1, 3)[0]
+        e, r = O.ubk_epsbk(uu)
+        dn = 0.5 * O.ubk_dns(uu) + 0.5 * dmix
+        want = [np.abs(dn - dmix).sum() / 16, r.mean(), r.max(),
+                e[2:, :].min() - e[:2, :].max(), (e[2, :] - e[1, :]).min()]
+        dmix = dn
+        np.testing.assert_allclose(dg[it], want, rtol=1e-12, atol=1e-14)
+    np.testing.assert_array_equal(u, uu)
+
+
+def test_make_dos_normalisation_and_peak():
+    """Gaussians normalised to 1: sum(DoS) de = NB NK occmax, sum(DoSocc) de =
+    NBocc NK occ; a single level peaks at its energy with height
+    occ/(w sqrt(pi))."""
+    rng = np.random.default_rng(2)
+    eps = np.sort(rng.uniform(-2.0, 1.0, (3, 4)), axis=0)
+    x, d, q = po.make_dos(eps, 2, 10000)
+    de = x[1] - x[0]
+    assert abs(d.sum() * de - 3 * 4 * 0.5) < 1e-9
+    assert abs(q.sum() * de - 2 * 4 * 0.5) < 1e-9
+    one = np.array([[0.25, 1.25]]).T  # NB=2, NK=1: range [0.25, 1.25]
+    x, d, q = po.make_dos(one, 1, 1000)
+    w = 5 * (x[1] - x[0])
+    i = np.argmin(abs(x - 0.25))
+    assert abs(q[i] - 2.0 / (w * math.sqrt(math.pi)) * math.exp(-((0.25 - x[i]) / w) ** 2)) < 1e-12
+    assert abs(x[0] - (0.25 - 0.1)) < 1e-15 and abs(x[-1] + (x[1] - x[0]) - (1.25 + 0.1)) < 1e-12
+
+
+def test_fourier_matches_fft_on_its_grid():
+    """At omega_i = 2 pi i / (NT dt) the transform is NT * ifft of the
+    filtered signal times dt (numpy's FFT, an independent algorithm)."""
+    nt, dt = 256, 0.05
+    t = dt * np.arange(nt)
+    rng = np.random.default_rng(1)
+    J, E, A = rng.standard_normal(nt), rng.standard_normal(nt), rng.standard_normal(nt)
+    nomega = 20
+    omegam = nomega * 2 * math.pi / (nt * dt)
+    out = po.fourier(t, J, 0.3, E, A, dt, nomega, omegam, window=0)
+    f = 1 - 3 * (t / t[-1]) ** 2 + 2 * (t / t[-1]) ** 3
+    for sig, col in (((J - 0.3) * f, 1), (E * f, 3), (A * f, 5)):
+        ref = np.fft.ifft(sig)[1:nomega + 1] * nt * dt
+        np.testing.assert_allclose(out[:, col] + 1j * out[:, col + 1], ref, rtol=0, atol=1e-12)
+    # window 1: sin^2 envelope over the first tpulse/dt steps only
+    tp = 7.0
+    out1 = po.fourier(t, J, 0.0, E, A, dt, nomega, omegam, window=1, tpulse=tp, nenvelope=2)
+    g = np.where(np.arange(nt) < int(tp / dt), np.sin(np.pi * t / tp) ** 2, 0.0)
+    ref = np.fft.ifft(J * g)[1:nomega + 1] * nt * dt
+    np.testing.assert_allclose(out1[:, 1] + 1j * out1[:, 2], ref, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(out[:, 0], (np.arange(nomega) + 1) * omegam / nomega, rtol=1e-15)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_taylor_step_is_the_matrix_series(variant):
+    """One RT_TE/TEPC/TEPC2 step equals the 4th-order Taylor polynomial of
+    the dense matrices H[u] built column by column from the oracle's H|w>."""
+    S = po.System(16, 2, 2, 2, 5.0)
+    vion = S.ion_potential([2.0], [1.0], [0.0])
+    O = po.Oracle(S, vion)
+    u0 = S.gram_schmidt(S.init_wf())
+    dt, aave = 0.05, 0.1
+    n = 16 * 2 * 2
+
+    def hmat(u):
+        cols = []
+        for k in range(n):
+            e = np.zeros(n, complex)
+            e[k] = 1.0
+            cols.append(O.ubkwbk_hwbk(u, e.reshape(S.shape, order="F"), aave).ravel(order="F"))
+        return np.array(cols).T
+
+    def series(H, x):
+        out, w = x.copy(), x.copy()
+        for k in range(1, 5):
+            w = -1j * dt * (H @ w) / k
+            out = out + w
+        return out
+
+    x0 = u0.ravel(order="F")
+    H0 = hmat(u0)
+    if variant == 0:
+        want = series(H0, x0)
+    else:
+        x1 = series(H0, x0)
+        if variant == 1:
+            want = series(0.5 * (H0 + hmat(x1.reshape(S.shape, order="F"))), x0)
+        else:
+            want = series(hmat((0.5 * (x0 + x1)).reshape(S.shape, order="F")), x0)
+    got = O.TE_step(u0, dt, aave, variant).ravel(order="F")
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
