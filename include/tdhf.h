@@ -138,6 +138,40 @@ int tdhf_gram_schmidt(tdhf_ctx* ctx, double* u);
 int tdhf_set_kernel_timing(tdhf_ctx* ctx, int enable);
 int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);
 
+/* RT_TE_ubk (variant 0), RT_TEPC_ubk (1), RT_TEPC2_ubk (2),
+ * src/RT_calculation.f90:71-151: one step of the 4th-order Taylor
+ * propagators with H frozen at the step start (TE), with a predictor-
+ * corrector average of H (TEPC) or H at the predictor midpoint (TEPC2), at
+ * field Aave; u in/out (NG*NB*NK complex). */
+int tdhf_te_step(tdhf_ctx* ctx, double* u, double Aave, int variant);
+
+/* GS_calculation (src/GS_calculation.f90:2-121) with the per-iteration
+ * columns of <sys_name>_GSinfo.out: diag[5*(iter-1) + ...] = mean |change of
+ * the mixed density dnsscf| (mixing 1/2), mean and max epserrorbk,
+ * fundamental gap, smallest same-k gap (DBL_MAX when NB == NBocc, where the
+ * reference's minval runs over an empty set); nscf iterations, no early
+ * stop.  etot_hist: nscf+1; epsbk/epserr: NB*NK of the final state. */
+int tdhf_gs_run_diag(tdhf_ctx* ctx, double* u, int nscf, int ncg, double* etot_hist,
+                     double* epsbk, double* epserr, double* diag);
+
+/* Make_DoS with GS_calculation's energy grid (src/GS_calculation.f90:93-97,
+ * 239-265): epsdos[i] = emin + (emax-emin) i/ndos with emin/emax the band
+ * extremes -/+ 10 % of their range, Gaussian width 5 (epsdos[1]-epsdos[0]);
+ * dos weighted by the k-point's largest occupation, dosocc by the band's
+ * (2/NK for ib < NBocc).  epsbk: NB*NK. */
+int tdhf_make_dos(int nb, int nk, int nbocc, const double* epsbk, int ndos, double* epsdos,
+                  double* dos, double* dosocc);
+
+/* FourierTr (window 0) / WindowFourierTr (window 1), src/RT_calculation.f90
+ * :246-351: out[7*i + ...] = omega_i = (i+1)/nomega*omegam, Re JF, Im JF,
+ * Re EF, Im EF, Re AF, Im AF with XF = sum_t exp(i omega t) filter(t) X(t) dt
+ * (X = J - j0, E, A); filter 0 = 1 - 3 (t/t_N)^2 + 2 (t/t_N)^3, filter 1 =
+ * sin(pi t/tpulse)^nenvelope on the first int(min(tpulse/dt, nt)) steps.
+ * time, J, E, A: nt values (time as Define_Afield: dt*(it-1)). */
+int tdhf_fourier(int nt, double dt, const double* time, const double* J, double j0,
+                 const double* E, const double* A, int nomega, double omegam, int window,
+                 double tpulse, int nenvelope, double* out);
+
 /* ------------------------------------------------------------------------
  * Differentiation.  No counterpart in the reference: a hand-written adjoint
  * of the RK4 propagation (RT_RK4_ubk, src/RT_calculation.f90:154-173, with

@@ -23,6 +23,7 @@
 #include "tdhf_kernels.h"
 #include "tdhf_gs.h"
 #include "tdhf_adjoint.h"
+#include "tdhf_post.h"
 
 using tdhf::zd;
 
@@ -811,3 +812,4 @@ int tdhf_kernel_stats(tdhf_ctx* c, long* launches, double* total_ms) {
 
 // batched trajectories (tdhf_batch_*)
 #include "tdhf_batch_impl.h"
+#include "tdhf_post_impl.h"

@@ -8,9 +8,16 @@
 ! computed by libtdhf.so through the iso_c_binding module tdhf_capi in place
 ! of GS_calculation and RT_calculation (src/mATTOHF.f90:104,114).  Output
 ! files follow the reference's names and formats:
-!   <sys_name>_GSinfo.out   iteration, Etot, Etot - Etot(prev)
-!   <sys_name>_band.out     ik, kx, residual, eps(:,ik)   (src/GS_calculation.f90:112-117)
+!   <sys_name>_GSinfo.out     per-iteration columns and the SCF summary
+!                             (src/GS_calculation.f90:22-24,48-51,61-91)
+!   <sys_name>_DoS.out        band table and Make_DoS curves (:93-110, 239-265)
+!   <sys_name>_band.out       ik, kx, residual, eps(:,ik)   (:112-117)
 !   <sys_name>_EtotNJEAt.out  8 columns, e20.12          (src/RT_calculation.f90:22-24,52-53)
+!   <sys_name>_JEAF.out, _JEAF_window.out, _sigmaeps.out (LR)
+!                             FourierTr / WindowFourierTr (:246-351)
+! TD_option 'TE', 'TEPC' or 'TEPC2' selects the reference's Taylor propagators
+! (src/RT_calculation.f90:71-151, commented out in its time loop :39-41)
+! instead of RK4.
 !------------------------------------------------------------------------------
 module drv_state
   implicit none
@@ -46,7 +53,10 @@ program mattohf_drv
   real(8), allocatable :: vion(:), zion(:), aion(:), kx(:), rx(:)
   real(8), allocatable :: etot_hist(:), epsbk(:,:), epserr(:,:)
   real(8), allocatable :: time(:), A(:), Aave(:), E(:), Etot(:), Nele(:), Jtot(:)
-  real(8) :: init3(3), bx
+  real(8), allocatable :: diag(:,:), epsdos(:), dos(:), dosocc(:), spec(:,:)
+  real(8) :: init3(3), bx, ecomps(4), etmp, ntmp, jtmp, Tau
+  integer, parameter :: NDoS = 10000
+  real(8), parameter :: Hartree = 27.2116d0, Atomtime = 0.02419d0
   integer(c_int) :: rc, niter
   integer :: ik, ib, it, ig, iseed, ii
   real(8) :: rnd, rxc
@@ -98,14 +108,66 @@ program mattohf_drv
   call tdhf_check(rc, ctx, 'tdhf_gram_schmidt')
 
   ! ground state
-  allocate(etot_hist(0:Nscf), epsbk(NB,NK), epserr(NB,NK))
-  rc = tdhf_gs_run(ctx, ubk, int(Nscf, c_int), int(NCG, c_int), gs_tol, etot_hist, epsbk, &
-       epserr, niter)
-  call tdhf_check(rc, ctx, 'tdhf_gs_run')
+  allocate(etot_hist(0:Nscf), epsbk(NB,NK), epserr(NB,NK), diag(5,max(Nscf,1)))
+  if (gs_tol > 0.d0) then
+    rc = tdhf_gs_run(ctx, ubk, int(Nscf, c_int), int(NCG, c_int), gs_tol, etot_hist, epsbk, &
+         epserr, niter)
+    call tdhf_check(rc, ctx, 'tdhf_gs_run')
+  else
+    rc = tdhf_gs_run_diag(ctx, ubk, int(Nscf, c_int), int(NCG, c_int), etot_hist, epsbk, &
+         epserr, diag)
+    call tdhf_check(rc, ctx, 'tdhf_gs_run_diag')
+    niter = Nscf
+  end if
+  rc = tdhf_observables(ctx, ubk, 0.d0, etmp, ecomps, ntmp, jtmp)
+  call tdhf_check(rc, ctx, 'tdhf_observables')
   open(10, file=trim(sys_name)//'_GSinfo.out', action='write')
-  write(10,'("#iter, Etotscf(iter), Etotscf(iter)-Etotscf(iter-1)")')
+  write(10,'("#iter, Etotscf(iter), Etoscf(iter)-Etotscf(iter-1), ,sum(abs(dnsscf(:,iter+1)-dnsscf(:,iter)))/dble(NG) &
+         sum(epserrorbk)/dble(NB*NK),maxval(epserrorbk),Fundamental gap, Optical Gap")')
   do it = 1, niter
-    write(10,'(2x,i5,100e20.12)') it, etot_hist(it), etot_hist(it) - etot_hist(it-1)
+    if (gs_tol > 0.d0) then
+      write(10,'(2x,i5,100e20.12)') it, etot_hist(it), etot_hist(it) - etot_hist(it-1)
+    else
+      write(10,'(2x,i5,100e20.12)') it, etot_hist(it), etot_hist(it) - etot_hist(it-1), diag(:,it)
+    end if
+  end do
+  write(10,'("#Ekin  at SCF:",f12.8)') ecomps(1)
+  write(10,'("#Eion  at SCF:",f12.8)') ecomps(2)
+  write(10,'("#EH    at SCF:",f12.8)') ecomps(3)
+  write(10,'("#Exc   at SCF:",f12.8)') ecomps(4)
+  write(10,'("#Min. value of the spectra: minval(epsbk)    =",f12.8," =",f12.8," [eV]")') &
+       minval(epsbk), minval(epsbk)*Hartree
+  write(10,'("#Max. value of the spectra: maxval(epsbk)    =",f12.8," =",f12.8," [eV]")') &
+       maxval(epsbk), maxval(epsbk)*Hartree
+  write(10,'("#Valence top: maxval(epsbk(:NBocc,:))        =",f12.8," =",f12.8," [eV]")') &
+       maxval(epsbk(:NBocc,:)), maxval(epsbk(:NBocc,:))*Hartree
+  ! the reference prints maxval here (src/GS_calculation.f90:84-85)
+  write(10,'("#Conduction bottom: minval(epsbk(NBocc+1:,:))=",f12.8," =",f12.8," [eV]")') &
+       maxval(epsbk(NBocc+1:,:)), maxval(epsbk(NBocc+1:,:))*Hartree
+  write(10,'("#Fundamental gap: ",f12.8," =",f12.8," [eV]")') &
+       minval(epsbk(NBocc+1:,:)) - maxval(epsbk(:NBocc,:)), &
+       (minval(epsbk(NBocc+1:,:)) - maxval(epsbk(:NBocc,:)))*Hartree
+  if (NB > NBocc) then
+    write(10,'("#Min. gap among same k-point: ",f12.8," =",f12.8," [eV]")') &
+         minval(epsbk(NBocc+1,:)-epsbk(NBocc,:)), (minval(epsbk(NBocc+1,:)-epsbk(NBocc,:)))*Hartree
+  else
+    write(10,'("#Min. gap among same k-point: ",f12.8," =",f12.8," [eV]")') &
+         huge(0.d0), huge(0.d0)
+  end if
+  close(10)
+  ! Make_DoS on the device, written as src/GS_calculation.f90:99-110
+  allocate(epsdos(NDoS), dos(NDoS), dosocc(NDoS))
+  rc = tdhf_make_dos(int(NB, c_int), int(NK, c_int), int(NBocc, c_int), epsbk, &
+       int(NDoS, c_int), epsdos, dos, dosocc)
+  call tdhf_check(rc, c_null_ptr, 'tdhf_make_dos')
+  open(10, file=trim(sys_name)//'_DoS.out', action='write')
+  write(10,*) '# ik,kx(ik),  sum(epserrorbk(:,ik))/dble(NB),  epsbk(:,ik)'
+  do ik = 1, NK
+    write(10,'("# ",i5,3e20.12,4x,1e20.12,4x,1000e20.12)') ik, kx(ik), sum(epserr(:,ik))/dble(NB), &
+         epsbk(:,ik)
+  end do
+  do ii = 1, NDoS
+    write(10,'(3e20.12e3)') epsdos(ii), dos(ii), dosocc(ii)
   end do
   close(10)
   open(10, file=trim(sys_name)//'_band.out', action='write')
@@ -120,8 +182,27 @@ program mattohf_drv
   ! real-time propagation
   allocate(time(NT), A(NT), Aave(NT), E(NT), Etot(NT), Nele(NT), Jtot(NT))
   call define_afield(time, A, Aave, E)
-  rc = tdhf_rt_run(ctx, ubk, int(NT, c_int), A, Aave, Etot, Nele, Jtot, init3)
-  call tdhf_check(rc, ctx, 'tdhf_rt_run')
+  select case (trim(TD_option))
+  case ('TE', 'TEPC', 'TEPC2')
+    ! the reference's Taylor propagators, one device step at a time
+    rc = tdhf_observables(ctx, ubk, 0.d0, init3(1), ecomps, init3(2), init3(3))
+    call tdhf_check(rc, ctx, 'tdhf_observables')
+    do it = 1, NT
+      if (trim(TD_option) == 'TE') then
+        rc = tdhf_te_step(ctx, ubk, Aave(it), 0_c_int)
+      else if (trim(TD_option) == 'TEPC') then
+        rc = tdhf_te_step(ctx, ubk, Aave(it), 1_c_int)
+      else
+        rc = tdhf_te_step(ctx, ubk, Aave(it), 2_c_int)
+      end if
+      call tdhf_check(rc, ctx, 'tdhf_te_step')
+      rc = tdhf_observables(ctx, ubk, A(it), Etot(it), ecomps, Nele(it), Jtot(it))
+      call tdhf_check(rc, ctx, 'tdhf_observables')
+    end do
+  case default
+    rc = tdhf_rt_run(ctx, ubk, int(NT, c_int), A, Aave, Etot, Nele, Jtot, init3)
+    call tdhf_check(rc, ctx, 'tdhf_rt_run')
+  end select
   open(10, file=trim(sys_name)//'_EtotNJEAt.out', action='write')
   write(10,'("#time(it), Etot(it)-Etot0, N(it)-N0, J(it), E(it), A(it)")')
   do it = 1, NT
@@ -130,9 +211,46 @@ program mattohf_drv
   end do
   close(10)
   write(*,'(" RT: ",i8," steps, Etot(NT) - Etot0 =",e14.6)') NT, Etot(NT) - init3(1)
+
+  ! FourierTr and WindowFourierTr on the device (src/RT_calculation.f90:64-65)
+  allocate(spec(7, Nomega))
+  rc = tdhf_fourier(int(NT, c_int), dt, time, Jtot, init3(3), E, A, int(Nomega, c_int), omegaM, &
+       0_c_int, 0.d0, int(nenvelope, c_int), spec)
+  call tdhf_check(rc, c_null_ptr, 'tdhf_fourier')
+  call write_jeaf(trim(sys_name)//'_JEAF.out')
+  if (field_option == 'LR') then
+    open(10, file=trim(sys_name)//'_sigmaeps.out', action='write')
+    write(10,'("#omega(iomega), sigma(iomega), eps(iomega)")')
+    do ii = 1, Nomega
+      block
+        complex(8) :: sig, eps
+        sig = -cmplx(spec(2,ii), spec(3,ii), kind=8)/dAc
+        eps = 4.d0*pi*(0.d0,1.d0)*sig/spec(1,ii)
+        write(10,'(20e20.12)') spec(1,ii), real(sig), aimag(sig), real(eps), aimag(eps)
+      end block
+    end do
+    close(10)
+  else
+    Tau = Tpulse/Atomtime
+    rc = tdhf_fourier(int(NT, c_int), dt, time, Jtot, init3(3), E, A, int(Nomega, c_int), &
+         omegaM, 1_c_int, Tau, int(nenvelope, c_int), spec)
+    call tdhf_check(rc, c_null_ptr, 'tdhf_fourier')
+    call write_jeaf(trim(sys_name)//'_JEAF_window.out')
+  end if
   rc = tdhf_destroy(ctx)
 
 contains
+
+  subroutine write_jeaf(fname)
+    character(*), intent(in) :: fname
+    integer :: iw
+    open(10, file=fname, action='write')
+    write(10,'("#omega(iomega), JF(iomega), EF(iomega), AF(iomega)")')
+    do iw = 1, Nomega
+      write(10,'(20e20.12)') spec(:, iw)
+    end do
+    close(10)
+  end subroutine write_jeaf
 
   ! Read_input, src/mATTOHF.f90:198-272 (one keyword per line, values on the
   ! next; unknown lines are skipped).  'gs_tol' is an addition of this driver.

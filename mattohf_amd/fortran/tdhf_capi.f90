@@ -102,6 +102,40 @@ module tdhf_capi
       complex(c_double_complex), intent(inout) :: u(*)
     end function
 
+    integer(c_int) function tdhf_te_step(ctx, u, Aave, variant) bind(C, name="tdhf_te_step")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      real(c_double), value :: Aave
+      integer(c_int), value :: variant
+    end function
+
+    integer(c_int) function tdhf_gs_run_diag(ctx, u, nscf, ncg, etot_hist, epsbk, epserr, diag) &
+         bind(C, name="tdhf_gs_run_diag")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(inout) :: u(*)
+      integer(c_int), value :: nscf, ncg
+      real(c_double), intent(out) :: etot_hist(*), epsbk(*), epserr(*), diag(*)
+    end function
+
+    integer(c_int) function tdhf_make_dos(nb, nk, nbocc, epsbk, ndos, epsdos, dos, dosocc) &
+         bind(C, name="tdhf_make_dos")
+      import :: c_int, c_double
+      integer(c_int), value :: nb, nk, nbocc, ndos
+      real(c_double), intent(in) :: epsbk(*)
+      real(c_double), intent(out) :: epsdos(*), dos(*), dosocc(*)
+    end function
+
+    integer(c_int) function tdhf_fourier(nt, dt, time, J, j0, E, A, nomega, omegam, window, &
+         tpulse, nenvelope, out) bind(C, name="tdhf_fourier")
+      import :: c_int, c_double
+      integer(c_int), value :: nt, nomega, window, nenvelope
+      real(c_double), value :: dt, j0, omegam, tpulse
+      real(c_double), intent(in) :: time(*), J(*), E(*), A(*)
+      real(c_double), intent(out) :: out(*)
+    end function
+
     ! adjoint pair: gradient of Etot(u_N) - eref with respect to A(1:nt) and u0
     integer(c_int) function tdhf_fwd(ctx, u, nt, A, eref, mem_budget, L, etot, nele, jtot) &
          bind(C, name="tdhf_fwd")

@@ -62,6 +62,10 @@ SIGNATURES = {
     "tdhf_bwd": (_i, [_vp, _vp, _vp]),
     "tdhf_adjoint_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
     "tdhf_fwd_batch": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _d, _vp, _vp, _vp, _vp]),
+    "tdhf_te_step": (_i, [_vp, _vp, _d, _i]),
+    "tdhf_gs_run_diag": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
+    "tdhf_make_dos": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp]),
+    "tdhf_fourier": (_i, [_i, _d, _vp, _vp, _d, _vp, _vp, _i, _d, _i, _d, _i, _vp]),
     "tdhf_bwd_batch": (_i, [_vp, _vp, _vp]),
     "tdhf_batch_create": (_i, [_PP, _i, _vp, _vp, C.POINTER(_vp)]),
     "tdhf_batch_destroy": (_i, [_vp]),
@@ -387,6 +391,24 @@ class TDHF:
                if want_u0 else None)
         return g, gus
 
+    def TE_step(self, ubk, Aave, variant=0):
+        """RT_TE_ubk (0), RT_TEPC_ubk (1), RT_TEPC2_ubk (2): one Taylor step."""
+        u = self._u(ubk, copy=True)
+        _chk(load().tdhf_te_step(self.h, _ptr(u), float(Aave), int(variant)), self.h)
+        return u
+
+    def GS_calculation_diag(self, ubk, nscf, ncg):
+        """GS_calculation with the per-iteration _GSinfo.out columns:
+        (u, etot_hist, epsbk, epserr, diag(nscf, 5))."""
+        u = self._u(ubk, copy=True)
+        eh = np.empty(nscf + 1)
+        eps = np.empty((self.nb, self.nk), order="F")
+        err = np.empty((self.nb, self.nk), order="F")
+        dg = np.empty((max(nscf, 1), 5))
+        _chk(load().tdhf_gs_run_diag(self.h, _ptr(u), int(nscf), int(ncg), _ptr(eh), _ptr(eps),
+                                     _ptr(err), _ptr(dg)), self.h)
+        return u, eh, eps, err, dg[:nscf]
+
     def adjoint_info(self):
         f, s = C.c_int(), C.c_int()
         _chk(load().tdhf_adjoint_info(self.h, C.byref(f), C.byref(s)), self.h)
@@ -503,6 +525,26 @@ class Batch:
 # ---------------------------------------------------------------------------
 # pulse parameterisation by piecewise-linear knots (the adjoint workload)
 # ---------------------------------------------------------------------------
+def make_dos(epsbk, nbocc, ndos=10000):
+    """Make_DoS on GS_calculation's energy grid (tdhf_make_dos): (epsdos, dos, dosocc)."""
+    e = np.asfortranarray(np.asarray(epsbk, dtype=np.float64))
+    nb, nk = e.shape
+    x, d, q = np.empty(ndos), np.empty(ndos), np.empty(ndos)
+    _chk(load().tdhf_make_dos(nb, nk, int(nbocc), _ptr(e), int(ndos), _ptr(x), _ptr(d), _ptr(q)))
+    return x, d, q
+
+
+def fourier(time, J, j0, E, A, dt, nomega, omegam, window=0, tpulse=0.0, nenvelope=2):
+    """FourierTr (window 0) / WindowFourierTr (window 1) on the device
+    (tdhf_fourier): (nomega, 7) rows omega, Re/Im JF, Re/Im EF, Re/Im AF."""
+    t, J, E, A = (np.ascontiguousarray(v, dtype=np.float64) for v in (time, J, E, A))
+    out = np.empty((int(nomega), 7))
+    _chk(load().tdhf_fourier(t.size, float(dt), _ptr(t), _ptr(J), float(j0), _ptr(E), _ptr(A),
+                             int(nomega), float(omegam), int(window), float(tpulse),
+                             int(nenvelope), _ptr(out)))
+    return out
+
+
 def knot_matrix(nt, dt, T, nknots):
     """P (nt x nknots): A[it] = sum_m P[it,m] c_m is the piecewise-linear
     interpolation of knots c_m at tau_m = m T/(nknots-1), times the sin^2
