@@ -131,6 +131,61 @@ __device__ __forceinline__ void group_reduce(const double* base, int stride, int
   }
 }
 
+// Exchange-and-add steps of wave_reduce_scatter for a pair of values (x, y)
+// over one lane bit: lanes with the bit clear return x summed over the lane
+// pair, lanes with the bit set return y summed over the pair.
+// Bit 5: v_permlane32_swap exchanges the upper half of x with the lower half
+// of y, so x' + y' is (x_lo + x_hi | y_lo + y_hi).
+__device__ __forceinline__ double pair_sum_bit5(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(x),
+                                                   (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(x),
+                                                   (unsigned)__double2hiint(y), false, false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// Bit 4: v_permlane16_swap exchanges the odd 16-lane rows of x with the even
+// rows of y.
+__device__ __forceinline__ double pair_sum_bit4(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x),
+                                                   (unsigned)__double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x),
+                                                   (unsigned)__double2hiint(y), false, false);
+  return __hiloint2double((int)hi[0], (int)lo[0]) + __hiloint2double((int)hi[1], (int)lo[1]);
+}
+// Bit 3: DPP row_ror:8 reads lane (i ^ 8) within each 16-lane row.
+__device__ __forceinline__ double pair_sum_bit3(double x, double y, bool bit) {
+  const double send = bit ? x : y, keep = bit ? y : x;
+  const int rl = __builtin_amdgcn_update_dpp(0, __double2loint(send), 0x128, 0xf, 0xf, false);
+  const int rh = __builtin_amdgcn_update_dpp(0, __double2hiint(send), 0x128, 0xf, 0xf, false);
+  return keep + __hiloint2double(rh, rl);
+}
+
+// Sums the NV values v[] of every lane over the groups of L lanes of a wave
+// (lane bits log2(L)..5) without LDS, as a reduce-scatter: each step hands
+// half of the values to the partner lane of one lane bit and keeps the other
+// half, summed.  Afterwards v[k], k < NV L / 64, holds the sum over the
+// wave's groups of value k + (NV L / 64) gw, where gw = lane / L.
+template <int L, int NV>
+__device__ __forceinline__ void wave_reduce_scatter(double* v) {
+  static_assert(L == 8 || L == 16, "lane groups of 8 or 16");
+  const int lane = threadIdx.x & 63;
+  static_for<0, NV / 2>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    v[k] = pair_sum_bit5(v[k], v[k + NV / 2]);
+  });
+  static_for<0, NV / 4>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    v[k] = pair_sum_bit4(v[k], v[k + NV / 4]);
+  });
+  if constexpr (L == 8) {
+    const bool b3 = (lane & 8) != 0;
+    static_for<0, NV / 8>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      v[k] = pair_sum_bit3(v[k], v[k + NV / 8], b3);
+    });
+  }
+}
+
 // Sets the issue priority (0..3) of the executing wave; p must be the same in
 // every lane of the wave.
 __device__ __forceinline__ void set_wave_priority(int p) {
@@ -307,15 +362,17 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // the multipliers of the current pair are loaded before that prefetch, so
   // waiting for them leaves the prefetch in flight.
   const int nitem = nk * a.nbocc;
-  auto fetch = [&](int item) {
-    const int iq = item / a.nbocc, jb = item - iq * a.nbocc;
+  // item = jb + nbocc iq; a group's items are ngrp apart
+  const int dq = ngrp / a.nbocc, dj = ngrp - dq * a.nbocc;
+  int cq = grp / a.nbocc, cj = grp - cq * a.nbocc;  // item of the current pass
+  auto fetch = [&](int iq, int jb) {
     const zd* urow = u + (size_t)NG * (jb + a.nb * iq);
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       un[m] = gload(urow + L * m + l);
     });
   };
-  if (grp < nitem) fetch(grp);
+  if (grp < nitem) fetch(cq, cj);
   // The two waves sharing a SIMD would otherwise finish far apart (the older
   // one is issued first whenever both are ready, and the other then runs the
   // rest of its items alone).  Each wave lowers its issue priority as it
@@ -332,8 +389,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       constexpr int m = decltype(mc)::value;
       uq[m] = un[m];
     });
-    const int iq = item / a.nbocc;
-    const double* vt = vtl + (size_t)(ik - iq + nk - 1) * NG + l * R;
+    const double* vt = vtl + (size_t)(ik - cq + nk - 1) * NG + l * R;
     double vtr[R];
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
@@ -341,7 +397,16 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     });
     // unconditional (the last pass re-reads its own item), so the wait for
     // vtr above counts exactly the loads issued after it
-    fetch(item + ngrp < nitem ? item + ngrp : item);
+    int nq = cq + dq, nj = cj + dj;
+    if (nj >= a.nbocc) {
+      nj -= a.nbocc;
+      ++nq;
+    }
+    if (item + ngrp >= nitem) {
+      nq = cq;
+      nj = cj;
+    }
+    fetch(nq, nj);
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       v[m] = zmulc(sW[L * m + l], uq[m]);
@@ -360,6 +425,8 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       acc[m].y = fma(v[m].x, uq[m].y, acc[m].y);
       acc[m].y = fma(v[m].y, uq[m].x, acc[m].y);
     });
+    cq = nq;
+    cj = nj;
   }
   set_wave_priority(0);
 
@@ -376,19 +443,52 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
-  // ---- one reduction over all groups through LDS (fixed order): each group
-  // parks V_F w (2 NG doubles) and its density sum (NG doubles) in its slot;
-  // the sums land in sVF and, right after it, sRho (sum |u|^2, without occ) ----
-  double* slotd = reinterpret_cast<double*>(T);
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    slotd[2 * (L * m + l)] = acc[m].x;
-    slotd[2 * (L * m + l) + 1] = acc[m].y;
-    slotd[2 * NG + L * m + l] = den[m];
-  });
-  lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 3 * NG, sPart,
-               reinterpret_cast<double*>(sVF), 1.0);
+  // ---- one reduction over all groups (fixed order) of V_F w and the density
+  // sum; the sums land in sVF and, right after it, sRho (sum |u|^2, without
+  // occ) ----
+  if constexpr (R % GPW == 0) {
+    // the groups of a wave are summed in registers, then the nw wave partials
+    // through LDS, each parked in its own wave's slots as [re | im | rho][x]
+    constexpr int NV = 3 * R, VF = NV / GPW;
+    double v3[NV];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v3[3 * m] = acc[m].x;
+      v3[3 * m + 1] = acc[m].y;
+      v3[3 * m + 2] = den[m];
+    });
+    wave_reduce_scatter<L, NV>(v3);
+    double* wp = reinterpret_cast<double*>(smem + (size_t)wave * GPW * SSZ);
+    static_for<0, VF>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const int vi = k + VF * gw;  // value 3 m + c of the groups, m = point
+      wp[(vi % 3) * NG + L * (vi / 3) + l] = v3[k];
+    });
+    lds_barrier();
+    for (int t = tid; t < 3 * NG; t += blockDim.x) {
+      const int c = t / NG, x = t - c * NG;
+      double sum = 0.0;
+      for (int w2 = 0; w2 < nw; ++w2)
+        sum += reinterpret_cast<const double*>(smem + (size_t)w2 * GPW * SSZ)[t];
+      if (c < 2)
+        reinterpret_cast<double*>(sVF)[2 * x + c] = sum;
+      else
+        sRho[x] = sum;
+    }
+  } else {
+    // each group parks V_F w (2 NG doubles) and its density sum (NG doubles)
+    // in its slot, then all threads add the slots
+    double* slotd = reinterpret_cast<double*>(T);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      slotd[2 * (L * m + l)] = acc[m].x;
+      slotd[2 * (L * m + l) + 1] = acc[m].y;
+      slotd[2 * NG + L * m + l] = den[m];
+    });
+    lds_barrier();
+    group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 3 * NG, sPart,
+                 reinterpret_cast<double*>(sVF), 1.0);
+  }
   if (tid < NG) sKH[tid] = khv;
   lds_barrier();
 
