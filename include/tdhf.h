@@ -245,7 +245,8 @@ int tdhf_batch_propagate(tdhf_batch* b, int nt, const double* A, const double* A
 int tdhf_batch_run(const tdhf_params* p, int ntraj, const int* nk, const double* vion,
                    double* u, int nt, const double* A, const double* Aave, double* etot,
                    double* nele, double* jtot, double* init3);
-/* Launch shape: waves per workgroup of the fused kernel (2, 4 or 8).
+/* Launch shape: waves per workgroup of the fused kernel (1, 2, 4 or 8, at
+ * least NG/64; default the smallest).
  * tdhf_batch_autotune times nt_trial steps for each candidate on a copy of the
  * state (the batch state is unchanged), keeps the fastest and remembers it for
  * batches of the same shape created later in the process. */

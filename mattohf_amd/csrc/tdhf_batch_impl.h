@@ -376,9 +376,11 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
   }
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
-  // launch shape: cached autotuner choice, else 4 waves per workgroup
+  // launch shape: cached autotuner choice, else the fewest waves per
+  // workgroup that cover the grid (one at NG <= 64: the groups of a single wave
+  // need no workgroup barrier, and eight such workgroups share a CU)
   auto ct = tune_cache().find(tune_key(b));
-  b->nw = ct != tune_cache().end() ? ct->second : std::max(4, (p->ng + 63) / 64);
+  b->nw = ct != tune_cache().end() ? ct->second : (p->ng + 63) / 64;
   {
     const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
                      : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
@@ -498,7 +500,7 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
   BHIP(b, hipMemcpyAsync(save, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
   int best = b->nw;
   double best_t = 1e300;
-  const int cand[3] = {2, 4, 8};
+  const int cand[4] = {1, 2, 4, 8};
   for (int nw : cand) {
     if (!batch_valid_nw(b, nw)) continue;
     b->nw = nw;

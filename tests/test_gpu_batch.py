@@ -40,7 +40,7 @@ def batch_case():
     return systems, us, np.array(vions), A, Aave
 
 
-@pytest.mark.parametrize("nw", [2, 4, 8])
+@pytest.mark.parametrize("nw", [1, 2, 4, 8])
 def test_batch_matches_single_runs(batch_case, nw):
     systems, us, vions, A, Aave = batch_case
     with T.Batch(NG, NB, NB, ACELL, DT, NKS, vions) as B:
@@ -76,7 +76,37 @@ def test_autotune_keeps_state(batch_case):
     with T.Batch(NG, NB, NB, ACELL, DT, NKS, vions) as B:
         B.upload(us)
         best = B.autotune(A[:, :30], Aave[:, :30])
-        assert best in (2, 4, 8) and B.waves() == best
+        assert best in (1, 2, 4, 8) and B.waves() == best
         after = B.download()
         for a, b in zip(after, us):
             assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("nw", [1, 2])
+def test_batch_ng64_matches_single_runs(nw):
+    # NG=64: the groups of a wave are summed in registers; with one wave per
+    # workgroup that is the whole reduction
+    ng, nb, a, dt, nt, nks = 64, 2, 10.0, 0.01, 30, [4, 2, 4]
+    us, vions, fields = [], [], []
+    for t, nk in enumerate(nks):
+        S = po.System(ng, nk, nb, nb, a)
+        rng = np.random.default_rng(300 + t)
+        u = rng.standard_normal(S.shape) + 1j * rng.standard_normal(S.shape)
+        us.append(S.gram_schmidt(np.asfortranarray(u)))
+        vions.append(S.ion_potential([8.0], [1.0], [0.0]))
+        _, A, Aave, _ = T.define_afield_lp(nt, dt, *T.pulse_lp_params(0.1 + 0.1 * t, 0.057, 0.3))
+        fields.append((A, Aave))
+    A = np.array([f[0] for f in fields])
+    Aave = np.array([f[1] for f in fields])
+    with T.Batch(ng, nb, nb, a, dt, nks, np.array(vions)) as B:
+        B.set_waves(nw)
+        B.upload(us)
+        e, n, j, i3 = B.propagate(A, Aave)
+        finals = B.download()
+    for t, nk in enumerate(nks):
+        with T.TDHF(ng, nk, nb, nb, a, dt, vions[t]) as G:
+            u1, e1, n1, j1, _ = G.RT_calculation(us[t], A[t], Aave[t])
+        np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * max(1.0, np.abs(e1).max()))
+        np.testing.assert_allclose(n[t], n1, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(j[t], j1, rtol=0, atol=1e-12 * max(1.0, np.abs(j1).max()))
+        np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
