@@ -252,17 +252,47 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
 
   // ---- one reduction over the groups (fixed order): exchange sum, density
   // and s land in sACC and, right after it, sRho and sS ----
-  double* slotd = reinterpret_cast<double*>(T);
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    slotd[2 * (L * m + l)] = acc[m].x;
-    slotd[2 * (L * m + l) + 1] = acc[m].y;
-    slotd[2 * NG + L * m + l] = den[m];
-    slotd[3 * NG + L * m + l] = sv[m];
-  });
-  lds_barrier();
-  group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 4 * NG, sPart,
-               reinterpret_cast<double*>(sACC), 1.0);
+  if constexpr ((4 * R) % GPW == 0) {
+    // the groups of a wave are summed in registers, then the nw wave partials
+    // through LDS, each parked in its own wave's slots as [re | im | rho | s][x]
+    constexpr int NV = 4 * R, VF = NV / GPW;
+    double v4[NV];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v4[4 * m] = acc[m].x;
+      v4[4 * m + 1] = acc[m].y;
+      v4[4 * m + 2] = den[m];
+      v4[4 * m + 3] = sv[m];
+    });
+    wave_reduce_scatter<L, NV>(v4);
+    double* wp = reinterpret_cast<double*>(smem + (size_t)wave * GPW * SSZ);
+    static_for<0, VF>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      const int vi = k + VF * gw;  // value 4 m + c of the groups, m = point
+      wp[(vi % 4) * NG + L * (vi / 4) + l] = v4[k];
+    });
+    lds_barrier();
+    double* outd = reinterpret_cast<double*>(sACC);
+    for (int t = tid; t < 4 * NG; t += blockDim.x) {
+      const int c = t / NG, x = t - c * NG;
+      double sum = 0.0;
+      for (int w2 = 0; w2 < nw; ++w2)
+        sum += reinterpret_cast<const double*>(smem + (size_t)w2 * GPW * SSZ)[t];
+      outd[c < 2 ? 2 * x + c : NG * c + x] = sum;
+    }
+  } else {
+    double* slotd = reinterpret_cast<double*>(T);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      slotd[2 * (L * m + l)] = acc[m].x;
+      slotd[2 * (L * m + l) + 1] = acc[m].y;
+      slotd[2 * NG + L * m + l] = den[m];
+      slotd[3 * NG + L * m + l] = sv[m];
+    });
+    lds_barrier();
+    group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 4 * NG, sPart,
+                 reinterpret_cast<double*>(sACC), 1.0);
+  }
   lds_barrier();
 
   // ---- circulant products over all threads: K_H * rho, K_H * s, K_2 * gbar,

@@ -160,6 +160,12 @@ __device__ __forceinline__ double pair_sum_bit3(double x, double y, bool bit) {
   return keep + __hiloint2double(rh, rl);
 }
 
+// Bit 2: ds_bpermute (__shfl_xor) from lane i ^ 4.
+__device__ __forceinline__ double pair_sum_bit2(double x, double y, bool bit) {
+  const double send = bit ? x : y, keep = bit ? y : x;
+  return keep + __shfl_xor(send, 4);
+}
+
 // Sums the NV values v[] of every lane over the groups of L lanes of a wave
 // (lane bits log2(L)..5) without LDS, as a reduce-scatter: each step hands
 // half of the values to the partner lane of one lane bit and keeps the other
@@ -167,7 +173,8 @@ __device__ __forceinline__ double pair_sum_bit3(double x, double y, bool bit) {
 // wave's groups of value k + (NV L / 64) gw, where gw = lane / L.
 template <int L, int NV>
 __device__ __forceinline__ void wave_reduce_scatter(double* v) {
-  static_assert(L == 8 || L == 16, "lane groups of 8 or 16");
+  static_assert(L == 4 || L == 8 || L == 16, "lane groups of 4, 8 or 16");
+  static_assert((NV * L) % 64 == 0, "values must split evenly over the groups");
   const int lane = threadIdx.x & 63;
   static_for<0, NV / 2>([&](auto kc) {
     constexpr int k = decltype(kc)::value;
@@ -177,11 +184,18 @@ __device__ __forceinline__ void wave_reduce_scatter(double* v) {
     constexpr int k = decltype(kc)::value;
     v[k] = pair_sum_bit4(v[k], v[k + NV / 4]);
   });
-  if constexpr (L == 8) {
+  if constexpr (L <= 8) {
     const bool b3 = (lane & 8) != 0;
     static_for<0, NV / 8>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
       v[k] = pair_sum_bit3(v[k], v[k + NV / 8], b3);
+    });
+  }
+  if constexpr (L <= 4) {
+    const bool b2 = (lane & 4) != 0;
+    static_for<0, NV / 16>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      v[k] = pair_sum_bit2(v[k], v[k + NV / 16], b2);
     });
   }
 }
