@@ -61,8 +61,9 @@ struct AdjArgs {
 };
 
 // Per-group LDS slot of adjoint_kernel (complex elements): the transpose
-// tile, then the group's exchange sum (2 NG doubles), density and s (NG
-// doubles each) for the single reduction round; 8 mod 16 elements as in
+// tile during the pair loop; afterwards the slot area holds the wave partials
+// of the reduction (4 NG doubles per wave), the circulant-product partials and
+// the field-derivative staging.  At least 2 NG elements, 8 mod 16 as in
 // apply_slot.
 template <int NG>
 __host__ __device__ constexpr int adjoint_slot() {
@@ -71,10 +72,10 @@ __host__ __device__ constexpr int adjoint_slot() {
 }
 
 // LDS of adjoint_kernel<NG> with nw waves (bytes): the groups' tiles (pair
-// transposes, then reduction slots, circulant-product partials and the field
-// derivative staging), y, gbar, the exchange sum and K_1 (complex NG each),
-// rho, s, K_H, K_2 (NG doubles each), a small scratch and the group_reduce
-// partials.
+// transposes, then reduction partials, circulant-product partials and the
+// field derivative staging), y, gbar, the exchange sum and K_1 (complex NG
+// each), rho, s, K_H, K_2 (NG doubles each), a small scratch and the chunk-sum
+// scratch.
 template <int NG>
 __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * adjoint_slot<NG>() * sizeof(zd) +
@@ -133,7 +134,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   double* sKH = sS + NG;
   double* sK2 = sKH + NG;
   double* sRed = sK2 + NG;
-  double* sPart = sRed + 16;  // max(blockDim.x, 4 NG) doubles for group_reduce
+  double* sPart = sRed + 16;  // max(blockDim.x, 4 NG) doubles (chunk sums)
   static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
                 "slot area too small for the epilogue scratch");
 
@@ -252,7 +253,8 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
 
   // ---- one reduction over the groups (fixed order): exchange sum, density
   // and s land in sACC and, right after it, sRho and sS ----
-  if constexpr ((4 * R) % GPW == 0) {
+  {
+    static_assert((4 * R) % GPW == 0, "four sums per point split over the groups");
     // the groups of a wave are summed in registers, then the nw wave partials
     // through LDS, each parked in its own wave's slots as [re | im | rho | s][x]
     constexpr int NV = 4 * R, VF = NV / GPW;
@@ -280,18 +282,6 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
         sum += reinterpret_cast<const double*>(smem + (size_t)w2 * GPW * SSZ)[t];
       outd[c < 2 ? 2 * x + c : NG * c + x] = sum;
     }
-  } else {
-    double* slotd = reinterpret_cast<double*>(T);
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      slotd[2 * (L * m + l)] = acc[m].x;
-      slotd[2 * (L * m + l) + 1] = acc[m].y;
-      slotd[2 * NG + L * m + l] = den[m];
-      slotd[3 * NG + L * m + l] = sv[m];
-    });
-    lds_barrier();
-    group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 4 * NG, sPart,
-                 reinterpret_cast<double*>(sACC), 1.0);
   }
   lds_barrier();
 

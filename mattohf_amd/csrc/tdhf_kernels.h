@@ -97,40 +97,6 @@ __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
   return g < NG / 2 ? (double)g * bx : (double)(g - NG) * bx;
 }
 
-// Sums, for each of nv values, the slots of ngrp groups (slot g at base +
-// g*stride doubles) into out[0..nv) times scale, using every thread of the
-// workgroup: each thread adds a strided subset of the groups with four
-// independent accumulators, then nv threads add the partials.  The order of
-// additions is fixed, so the result is the same on every run.  part must hold
-// max(blockDim.x, nv) doubles.  Contains an LDS barrier: call from uniform
-// control flow.
-__device__ __forceinline__ void group_reduce(const double* base, int stride, int ngrp, int nv,
-                                             double* part, double* out, double scale) {
-  const int nthr = blockDim.x;
-  int P = nthr / nv;
-  if (P < 1) P = 1;
-  if (P > ngrp) P = ngrp;
-  for (int t = threadIdx.x; t < nv * P; t += nthr) {
-    const int idx = t % nv, p = t / nv;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int g = p;
-    for (; g + 3 * P < ngrp; g += 4 * P) {
-      s0 += base[(size_t)g * stride + idx];
-      s1 += base[(size_t)(g + P) * stride + idx];
-      s2 += base[(size_t)(g + 2 * P) * stride + idx];
-      s3 += base[(size_t)(g + 3 * P) * stride + idx];
-    }
-    for (; g < ngrp; g += P) s0 += base[(size_t)g * stride + idx];
-    part[t] = (s0 + s1) + (s2 + s3);
-  }
-  lds_barrier();
-  for (int idx = threadIdx.x; idx < nv; idx += nthr) {
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) s += part[p * nv + idx];
-    out[idx] = scale * s;
-  }
-}
-
 // Exchange-and-add steps of wave_reduce_scatter for a pair of values (x, y)
 // over one lane bit: lanes with the bit clear return x summed over the lane
 // pair, lanes with the bit set return y summed over the pair.
@@ -211,15 +177,16 @@ __device__ __forceinline__ void set_wave_priority(int p) {
   }
 }
 
-// Scratch doubles after the fixed LDS arrays of apply_kernel: the
-// group_reduce partials of its 3 NG values, max(threads, 3 NG).
+// Scratch doubles after the fixed LDS arrays of apply_kernel (the chunk sums
+// of the observables), max(threads, 3 NG).
 __host__ __device__ constexpr size_t apply_part_doubles(int nw, int NG) {
   return (size_t)64 * nw > (size_t)3 * NG ? (size_t)64 * nw : (size_t)3 * NG;
 }
 
 // Per-group LDS slot of apply_kernel (complex elements): the transpose tile
-// during the pair loop, then the group's V_F w (2 NG doubles) and density
-// (NG doubles) for the single reduction round.  Rounded up to 8 mod 16
+// during the pair loop; afterwards the slot area holds the wave partials of
+// the reduction (3 NG doubles per wave), the circulant-product partials and
+// the observables staging.  At least 3 NG / 2 elements, rounded up to 8 mod 16
 // elements, so that consecutive groups start half a bank cycle (32 of the 64
 // four-byte banks) apart and the transposes of two groups served in one LDS
 // pass do not collide.
@@ -460,10 +427,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // ---- one reduction over all groups (fixed order) of V_F w and the density
   // sum; the sums land in sVF and, right after it, sRho (sum |u|^2, without
   // occ) ----
-  if constexpr (R % GPW == 0) {
-    // the groups of a wave are summed in registers, then the nw wave partials
-    // through LDS, each parked in its own wave's slots as [re | im | rho][x]
-    constexpr int NV = 3 * R, VF = NV / GPW;
+  {
+    // the groups of a wave are summed in registers (values padded with zeros
+    // to a multiple of the group count), then the nw wave partials through
+    // LDS, each parked in its own wave's slots as [re | im | rho][x]
+    constexpr int NV = (3 * R + GPW - 1) / GPW * GPW, VF = NV / GPW;
     double v3[NV];
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
@@ -471,12 +439,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       v3[3 * m + 1] = acc[m].y;
       v3[3 * m + 2] = den[m];
     });
+    static_for<3 * R, NV>([&](auto kc) { v3[decltype(kc)::value] = 0.0; });
     wave_reduce_scatter<L, NV>(v3);
     double* wp = reinterpret_cast<double*>(smem + (size_t)wave * GPW * SSZ);
     static_for<0, VF>([&](auto kc) {
       constexpr int k = decltype(kc)::value;
       const int vi = k + VF * gw;  // value 3 m + c of the groups, m = point
-      wp[(vi % 3) * NG + L * (vi / 3) + l] = v3[k];
+      if (vi < 3 * R) wp[(vi % 3) * NG + L * (vi / 3) + l] = v3[k];
     });
     lds_barrier();
     for (int t = tid; t < 3 * NG; t += blockDim.x) {
@@ -489,19 +458,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       else
         sRho[x] = sum;
     }
-  } else {
-    // each group parks V_F w (2 NG doubles) and its density sum (NG doubles)
-    // in its slot, then all threads add the slots
-    double* slotd = reinterpret_cast<double*>(T);
-    static_for<0, R>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      slotd[2 * (L * m + l)] = acc[m].x;
-      slotd[2 * (L * m + l) + 1] = acc[m].y;
-      slotd[2 * NG + L * m + l] = den[m];
-    });
-    lds_barrier();
-    group_reduce(reinterpret_cast<const double*>(smem), 2 * SSZ, ngrp, 3 * NG, sPart,
-                 reinterpret_cast<double*>(sVF), 1.0);
   }
   if (tid < NG) sKH[tid] = khv;
   lds_barrier();
