@@ -172,6 +172,13 @@ int tdhf_fourier(int nt, double dt, const double* time, const double* J, double 
                  const double* E, const double* A, int nomega, double omegam, int window,
                  double tpulse, int nenvelope, double* out);
 
+/* FP64 FMA throughput of the device (no counterpart in the reference: a
+ * measurement probe for the benchmark).  Runs iters rounds of 8 independent
+ * fused multiply-adds per thread on 1024 x 256 threads, once to warm up and
+ * three times timed; *tflops = 2 * 8 * iters * 262144 / (fastest time),
+ * *ms = that time (may be NULL). */
+int tdhf_fp64_fma_probe(int iters, double* tflops, double* ms);
+
 /* ------------------------------------------------------------------------
  * Differentiation.  No counterpart in the reference: a hand-written adjoint
  * of the RK4 propagation (RT_RK4_ubk, src/RT_calculation.f90:154-173, with

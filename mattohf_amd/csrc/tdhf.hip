@@ -813,3 +813,4 @@ int tdhf_kernel_stats(tdhf_ctx* c, long* launches, double* total_ms) {
 // batched trajectories (tdhf_batch_*)
 #include "tdhf_batch_impl.h"
 #include "tdhf_post_impl.h"
+#include "tdhf_probe.h"

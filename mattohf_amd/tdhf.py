@@ -66,6 +66,7 @@ SIGNATURES = {
     "tdhf_gs_run_diag": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
     "tdhf_make_dos": (_i, [_i, _i, _i, _vp, _i, _vp, _vp, _vp]),
     "tdhf_fourier": (_i, [_i, _d, _vp, _vp, _d, _vp, _vp, _i, _d, _i, _d, _i, _vp]),
+    "tdhf_fp64_fma_probe": (_i, [_i, C.POINTER(_d), C.POINTER(_d)]),
     "tdhf_bwd_batch": (_i, [_vp, _vp, _vp]),
     "tdhf_batch_create": (_i, [_PP, _i, _vp, _vp, C.POINTER(_vp)]),
     "tdhf_batch_destroy": (_i, [_vp]),
@@ -543,6 +544,14 @@ def fourier(time, J, j0, E, A, dt, nomega, omegam, window=0, tpulse=0.0, nenvelo
                              int(nomega), float(omegam), int(window), float(tpulse),
                              int(nenvelope), _ptr(out)))
     return out
+
+
+def fp64_fma_probe(iters=400000):
+    """Sustained FP64 FMA rate of the device in TFLOP/s and the timed launch's
+    milliseconds (tdhf_fp64_fma_probe)."""
+    tf, ms = C.c_double(), C.c_double()
+    _chk(load().tdhf_fp64_fma_probe(int(iters), C.byref(tf), C.byref(ms)))
+    return tf.value, ms.value
 
 
 def knot_matrix(nt, dt, T, nknots):

@@ -1,7 +1,8 @@
 """Taylor propagators, GS diagnostics, Make_DoS and the spectra through the
 C-ABI (tdhf_te_step, tdhf_gs_run_diag, tdhf_make_dos, tdhf_fourier) against
 the oracle (oracle/tdhf_oracle.c: orc_te_step, orc_gs_run_diag, orc_make_dos,
-orc_fourier), on seeded inputs."""
+orc_fourier), on seeded inputs; and the FP64 FMA probe the benchmark reads
+(tdhf_fp64_fma_probe)."""
 import numpy as np
 import pytest
 
@@ -76,3 +77,12 @@ def test_post_argument_errors():
                   tpulse=0.0)
     with pytest.raises(T.TDHFError):
         T.make_dos(np.zeros((2, 2)), 3, 100)
+
+
+def test_fp64_fma_probe():
+    # the sustained FP64 FMA rate: positive, below the 78.6 TFLOP/s the
+    # published peak gives, and the timed launch long enough to be measured
+    tf, ms = T.fp64_fma_probe(20000)
+    assert 1.0 < tf < 80.0 and ms > 0.0
+    with pytest.raises(T.TDHFError):
+        T.fp64_fma_probe(0)
