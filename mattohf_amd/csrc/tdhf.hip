@@ -165,10 +165,25 @@ int choose_nw(int ng, int nk, int nbocc) {
   return nw;
 }
 
+// Compute units of the current device (queried once).
+int device_cus() {
+  static const int n = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    return v;
+  }();
+  return n;
+}
+
 template <int NG>
 void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int ntraj, int nw, size_t lds,
                     hipStream_t s) {
-  hipLaunchKernelGGL(tdhf::apply_kernel<NG>, dim3(rows, ntraj), dim3(64 * nw), lds, s, a);
+  tdhf::ApplyArgs b = a;
+  // wave-priority balancing pays only when the grid is one workgroup per CU
+  b.balance = (long)rows * ntraj <= device_cus() ? 1 : 0;
+  hipLaunchKernelGGL(tdhf::apply_kernel<NG>, dim3(rows, ntraj), dim3(64 * nw), lds, s, b);
 }
 
 // rows workgroups per trajectory; ntraj > 1 for a uniform batch (grid y)

@@ -90,6 +90,7 @@ struct ApplyArgs {
   size_t tstride;
   int fstride;
   size_t ostride;
+  int balance;  // 1: balance the waves of a SIMD by issue priority (set by the launcher)
 };
 
 __device__ __forceinline__ double gvalue(int g, int NG, double bx) {
@@ -354,17 +355,20 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     });
   };
   if (grp < nitem) fetch(cq, cj);
-  // The two waves sharing a SIMD would otherwise finish far apart (the older
+  // When the whole grid is one workgroup per CU, the two waves sharing a SIMD
+  // belong to this workgroup and would otherwise finish far apart (the older
   // one is issued first whenever both are ready, and the other then runs the
-  // rest of its items alone).  Each wave lowers its issue priority as it
-  // passes quarters of its items, so a wave that falls behind is issued first
-  // until it catches up.
+  // rest of its items alone), while the workgroup waits for the later one.
+  // With a.balance each wave lowers its issue priority as it passes quarters
+  // of its items, so a wave that falls behind is issued first until it
+  // catches up.  With more workgroups than CUs the partner wave belongs to
+  // another workgroup and the default (oldest first) order is faster.
   const int wave_items = (nitem - wave * GPW + ngrp - 1) / ngrp;  // of its first group
   lds_barrier();  // sW visible
 
   int it = 0;
   for (int item = grp; item < nitem; item += ngrp, ++it) {
-    set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
+    if (a.balance) set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
     zd uq[R], v[R];
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
@@ -409,7 +413,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     cq = nq;
     cj = nj;
   }
-  set_wave_priority(0);
+  if (a.balance) set_wave_priority(0);
 
   // epilogue operands: loaded now, used after the reduction (whose barriers
   // order LDS only, so these loads stay in flight meanwhile)
