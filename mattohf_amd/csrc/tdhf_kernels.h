@@ -476,7 +476,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     if (p < P) {
       double vh = 0.0;
       zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
-      for (int y = p; y < NG; y += P) {
+      auto term = [&](int y) {
         const int d = (x - y) & (NG - 1);
         const zd wy = sW[y];
         const double k2 = sK2[d];
@@ -488,6 +488,14 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         c1.x = fma(-k1.y, wy.y, c1.x);
         c1.y = fma(k1.x, wy.y, c1.y);
         c1.y = fma(k1.y, wy.x, c1.y);
+      };
+      if (P == 1) {
+        // one-wave workgroups (batches): NG serial terms, unrolled so that
+        // several terms' LDS reads are in flight
+#pragma unroll 8
+        for (int y = 0; y < NG; ++y) term(y);
+      } else {
+        for (int y = p; y < NG; y += P) term(y);
       }
       double* q = reinterpret_cast<double*>(smem) + 5 * (size_t)tid;
       q[0] = vh;
