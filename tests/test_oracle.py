@@ -90,10 +90,11 @@ def test_k_vck_against_quadrature(ng, nk, a):
     assert M in (20 * ng * nk, 20 * ng * nk + 1)
     dx = a / ng / 20.0
     Lp = M * dx
-    f = lambda x, q: math.cos(q * x) / math.sqrt(x * x + 1.0)
     fp = lambda x, q: (-q * math.sin(q * x) * (x * x + 1.0) - x * math.cos(q * x)) / (x * x + 1.0) ** 1.5
     for q in (0.0, 0.3, 1.7, 5.0, 12.0):
-        exact = 2.0 * integrate.quad(f, 0.0, Lp, args=(q,), limit=2000, epsabs=1e-14,
+        # QUADPACK's cos-weighted rule (QAWO) for the oscillatory integrand
+        exact = 2.0 * integrate.quad(lambda x: 1.0 / math.sqrt(x * x + 1.0), 0.0, Lp,
+                                     weight="cos", wvar=q, limit=2000, epsabs=1e-12,
                                      epsrel=1e-13)[0]
         # trapezoid = integral + dx^2/12 (f'(L) - f'(0)) + O(dx^4), times 2
         em = 2.0 * dx * dx / 12.0 * (fp(Lp, q) - fp(0.0, q))
