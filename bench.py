@@ -360,7 +360,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--timing-steps", type=int, default=25)
+    ap.add_argument("--timing-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--batch-traj", type=int, default=256)
@@ -411,13 +411,21 @@ def main():
     wall = t1 - t0
     value = K / wall
 
-    # per-launch duration of the fused apply kernel (HIP events on its stream)
-    G.set_kernel_timing(True)
+    # duration of the fused apply kernel: HIP events on its stream around each
+    # graph-replayed block of steps (4 launches per step, no per-launch events)
+    G.set_kernel_timing(True, block=True)
     ts = args.timing_steps
     G.propagate(A[W + K:W + K + ts], Aave[W + K:W + K + ts])
     launches, total_ms = G.kernel_stats()
     G.set_kernel_timing(False)
     avg_s = total_ms / launches / 1e3
+    # the same with an event pair around every launch (graph replay off): an
+    # upper bound, each pair adds its own overhead to the interval
+    G.set_kernel_timing(True)
+    te = min(25, ts)
+    G.propagate(A[W + K:W + K + te], Aave[W + K:W + K + te])
+    launches_ev, total_ms_ev = G.kernel_stats()
+    G.set_kernel_timing(False)
     f_launch = flops_per_step(cfg["ng"], cfg["nk"], cfg["nb"], cfg["nbocc"]) / 4.0
     achieved = f_launch / avg_s / 1e12
 
@@ -462,6 +470,9 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "kernel": "apply_kernel<64> (fused exchange + H + RK4 stage)",
             "avg_launch_us": avg_s * 1e6,
+            "avg_launch_us_method": "HIP events on the kernel's stream around graph-replayed "
+                                    "blocks of steps, 4 launches per step",
+            "avg_launch_us_per_launch_events": total_ms_ev / launches_ev * 1e3,
             "flop_per_launch": f_launch,
             "timed_launches": launches,
         },

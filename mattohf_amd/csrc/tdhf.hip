@@ -66,8 +66,11 @@ struct tdhf_ctx {
   // launch shape of the apply kernel
   int nw = 8;
   size_t lds = 0;
-  // kernel timing
+  // kernel timing: per apply launch (timing) or per graph-replayed block of
+  // steps (block_timing, block_launches[i] apply launches in event pair i)
   bool timing = false;
+  bool block_timing = false;
+  std::vector<long> block_launches;
   long launches = 0;
   double total_ms = 0.0;
   std::vector<hipEvent_t> ev_pool;
@@ -187,18 +190,24 @@ void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int ntraj, int nw, size_
 }
 
 // rows workgroups per trajectory; ntraj > 1 for a uniform batch (grid y)
+// Two events of the context's pool for one timed interval.
+int take_events(tdhf_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  if (c->ev_used + 2 > c->ev_pool.size()) {
+    for (int i = 0; i < 64; ++i) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->ev_pool.push_back(e);
+    }
+  }
+  *e0 = c->ev_pool[c->ev_used++];
+  *e1 = c->ev_pool[c->ev_used++];
+  return 0;
+}
+
 int launch_apply(tdhf_ctx* c, const tdhf::ApplyArgs& a, int rows, int ntraj = 1) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (c->timing) {
-    if (c->ev_used + 2 > c->ev_pool.size()) {
-      for (int i = 0; i < 64; ++i) {
-        hipEvent_t e;
-        HIPCHK(c, hipEventCreate(&e));
-        c->ev_pool.push_back(e);
-      }
-    }
-    e0 = c->ev_pool[c->ev_used++];
-    e1 = c->ev_pool[c->ev_used++];
+    CHKRET(take_events(c, &e0, &e1));
     HIPCHK(c, hipEventRecord(e0, c->stream));
   }
   switch (c->ng) {
@@ -215,14 +224,15 @@ int launch_apply(tdhf_ctx* c, const tdhf::ApplyArgs& a, int rows, int ntraj = 1)
 
 // Collects the timed launches recorded so far (after a stream sync).
 int harvest_timing(tdhf_ctx* c) {
-  if (!c->timing) return 0;
+  if (!c->timing && !c->block_timing) return 0;
   for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
     c->total_ms += ms;
-    c->launches += 1;
+    c->launches += c->block_timing ? c->block_launches[i / 2] : 1;
   }
   c->ev_used = 0;
+  c->block_launches.clear();
   return 0;
 }
 
@@ -339,9 +349,20 @@ int run_steps(tdhf_ctx* c, int nt) {
       c->graph_steps = kGraphSteps;
       std::memcpy(c->graph_key, key, sizeof key);
     }
+    hipEvent_t b0 = nullptr, b1 = nullptr;
+    long blk = 0;
+    if (c->block_timing && nt - done >= c->graph_steps) {
+      CHKRET(take_events(c, &b0, &b1));
+      HIPCHK(c, hipEventRecord(b0, c->stream));
+    }
     while (nt - done >= c->graph_steps) {
       HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
       done += c->graph_steps;
+      blk += 4L * c->graph_steps;  // apply launches (4 RK4 stages per step)
+    }
+    if (b0) {
+      HIPCHK(c, hipEventRecord(b1, c->stream));
+      c->block_launches.push_back(blk);
     }
   }
   if (done < nt) {
@@ -808,8 +829,11 @@ int tdhf_gram_schmidt(tdhf_ctx* c, double* u) {
 
 int tdhf_set_kernel_timing(tdhf_ctx* c, int enable) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
+  if (enable < 0 || enable > 2) return fail(c, TDHF_EINVAL, "timing mode %d (0, 1 or 2)", enable);
   CHKRET(sync(c));
-  c->timing = enable != 0;
+  c->timing = enable == 1;
+  c->block_timing = enable == 2;
+  c->block_launches.clear();
   c->launches = 0;
   c->total_ms = 0.0;
   return 0;

@@ -415,8 +415,11 @@ class TDHF:
         _chk(load().tdhf_adjoint_info(self.h, C.byref(f), C.byref(s)), self.h)
         return bool(f.value), s.value
 
-    def set_kernel_timing(self, on):
-        _chk(load().tdhf_set_kernel_timing(self.h, 1 if on else 0), self.h)
+    def set_kernel_timing(self, on, block=False):
+        """on: time the fused apply kernel with HIP events; block: one event pair
+        per graph-replayed block of steps (4 launches per step) instead of one
+        per launch."""
+        _chk(load().tdhf_set_kernel_timing(self.h, (2 if block else 1) if on else 0), self.h)
 
     def kernel_stats(self):
         n, ms = C.c_long(), C.c_double()

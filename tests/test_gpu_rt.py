@@ -81,3 +81,27 @@ def test_rt_chunked_equals_single(system):
         js.append(G.ubk_Je(u, A[it]))
     np.testing.assert_allclose(np.array(js), j1, rtol=1e-12, atol=1e-16)
     np.testing.assert_allclose(u, G.RT_calculation(ugs, A, Aave)[0], rtol=0, atol=1e-14)
+
+
+def test_kernel_timing_modes(system):
+    # block mode: one event pair per graph-replayed block, 4 apply launches per
+    # step; per-launch mode: every apply launch (no graph); results unchanged
+    G, ugs = system["G"], system["ugs"]
+    nt = 50
+    _, A, Aave, _ = T.define_afield_lp(nt, DT, *T.pulse_lp_params(0.2, 0.057, 100.0))
+    res = []
+    for mode in (dict(on=False), dict(on=True, block=True), dict(on=True)):
+        G.upload(ugs)
+        G.set_kernel_timing(**mode)
+        e, n, j, _ = G.propagate(A, Aave)
+        launches, ms = G.kernel_stats()
+        G.set_kernel_timing(False)
+        res.append((e, j, launches, ms))
+    assert res[0][2] == 0
+    assert res[1][2] == 4 * nt and res[1][3] > 0.0
+    assert res[2][2] >= 4 * nt and res[2][3] > 0.0
+    for e, j, _, _ in res[1:]:
+        np.testing.assert_allclose(e, res[0][0], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(j, res[0][1], rtol=0, atol=1e-14)
+    with pytest.raises(T.TDHFError):
+        T._chk(T.load().tdhf_set_kernel_timing(G.h, 3), G.h)
