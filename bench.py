@@ -37,7 +37,7 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per apply_kernel<64> launch of the plain workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE per dispatch scaled by a plain 1 GiB device copy
 # (FETCH x2.0, WRITE x1.0): profiles/apply_kernel_ng64_nk64_nb4_counters.md
-TRAFFIC_KIB = {"fetch_kib": 1567.3, "fetch_scale": 2.0, "write_kib": 440.3, "write_scale": 1.0}
+TRAFFIC_KIB = {"fetch_kib": 1574.8, "fetch_scale": 2.0, "write_kib": 438.8, "write_scale": 1.0}
 
 CFG = dict(acell=10.0, ng=64, nk=64, nb=4, nbocc=4, z=8.0, alpha=1.0, dt=0.01, A0=0.3,
            omega=0.057, T=400.0)
@@ -411,13 +411,15 @@ def main():
     wall = t1 - t0
     value = K / wall
 
-    # duration of the fused apply kernel: HIP events on its stream around each
-    # graph-replayed block of steps (4 launches per step, no per-launch events)
+    # duration of the fused apply kernel: HIP events on its stream around the
+    # graph-replayed steps of one call (4 launches per step, no per-launch events)
     G.set_kernel_timing(True, block=True)
     ts = args.timing_steps
     G.propagate(A[W + K:W + K + ts], Aave[W + K:W + K + ts])
     launches, total_ms = G.kernel_stats()
     G.set_kernel_timing(False)
+    if launches < 1:
+        raise SystemExit("bench.py: no timed apply launches (--timing-steps must be >= 1)")
     avg_s = total_ms / launches / 1e3
     # the same with an event pair around every launch (graph replay off): an
     # upper bound, each pair adds its own overhead to the interval
@@ -470,8 +472,8 @@ def main():
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "kernel": "apply_kernel<64> (fused exchange + H + RK4 stage)",
             "avg_launch_us": avg_s * 1e6,
-            "avg_launch_us_method": "HIP events on the kernel's stream around graph-replayed "
-                                    "blocks of steps, 4 launches per step",
+            "avg_launch_us_method": "HIP events on the kernel's stream around the "
+                                    "graph-replayed steps of one call, 4 launches per step",
             "avg_launch_us_per_launch_events": total_ms_ev / launches_ev * 1e3,
             "flop_per_launch": f_launch,
             "timed_launches": launches,

@@ -134,9 +134,9 @@ int tdhf_gram_schmidt(tdhf_ctx* ctx, double* u);
 /* Per-kernel timing for measurement, with HIP events on the stream the
  * dominant kernel (the fused H-apply/RK4-stage kernel) is launched on.
  * enable = 1: every launch is bracketed by an event pair (graph replay off);
- * enable = 2: each graph-replayed block of RK4 steps in tdhf_rt_propagate is
- * bracketed by one event pair and counts 4 launches per step (no per-launch
- * event overhead); 0: off.  tdhf_kernel_stats returns the number of timed
+ * enable = 2: the RK4 steps of each tdhf_rt_propagate call (graph-replayed
+ * chunks and remainder) are bracketed by one event pair and count 4 launches
+ * per step (no per-launch event overhead); 0: off.  tdhf_kernel_stats returns the number of timed
  * launches and their summed duration in milliseconds since the last reset. */
 int tdhf_set_kernel_timing(tdhf_ctx* ctx, int enable);
 int tdhf_kernel_stats(tdhf_ctx* ctx, long* launches, double* total_ms);

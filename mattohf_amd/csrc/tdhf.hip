@@ -321,7 +321,22 @@ int enqueue_steps(tdhf_ctx* c, int j0, int n) {
 
 constexpr int kGraphSteps = 25;
 
+int run_steps_inner(tdhf_ctx* c, int nt);
+
+// nt RK4 steps; with block timing the whole sequence (graph-replayed chunks and
+// the remainder, 4 apply launches per step) is one timed interval.
 int run_steps(tdhf_ctx* c, int nt) {
+  if (!c->block_timing || nt < 1) return run_steps_inner(c, nt);
+  hipEvent_t b0 = nullptr, b1 = nullptr;
+  CHKRET(take_events(c, &b0, &b1));
+  HIPCHK(c, hipEventRecord(b0, c->stream));
+  CHKRET(run_steps_inner(c, nt));
+  HIPCHK(c, hipEventRecord(b1, c->stream));
+  c->block_launches.push_back(4L * nt);
+  return 0;
+}
+
+int run_steps_inner(tdhf_ctx* c, int nt) {
   int done = 0;
   const bool use_graph = !c->timing && nt >= kGraphSteps;
   if (use_graph) {
@@ -349,20 +364,9 @@ int run_steps(tdhf_ctx* c, int nt) {
       c->graph_steps = kGraphSteps;
       std::memcpy(c->graph_key, key, sizeof key);
     }
-    hipEvent_t b0 = nullptr, b1 = nullptr;
-    long blk = 0;
-    if (c->block_timing && nt - done >= c->graph_steps) {
-      CHKRET(take_events(c, &b0, &b1));
-      HIPCHK(c, hipEventRecord(b0, c->stream));
-    }
     while (nt - done >= c->graph_steps) {
       HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
       done += c->graph_steps;
-      blk += 4L * c->graph_steps;  // apply launches (4 RK4 stages per step)
-    }
-    if (b0) {
-      HIPCHK(c, hipEventRecord(b1, c->stream));
-      c->block_launches.push_back(blk);
     }
   }
   if (done < nt) {

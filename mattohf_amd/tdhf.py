@@ -417,8 +417,8 @@ class TDHF:
 
     def set_kernel_timing(self, on, block=False):
         """on: time the fused apply kernel with HIP events; block: one event pair
-        per graph-replayed block of steps (4 launches per step) instead of one
-        per launch."""
+        around the steps of each propagate call (4 launches per step) instead of
+        one per launch."""
         _chk(load().tdhf_set_kernel_timing(self.h, (2 if block else 1) if on else 0), self.h)
 
     def kernel_stats(self):

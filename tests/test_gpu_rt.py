@@ -84,8 +84,8 @@ def test_rt_chunked_equals_single(system):
 
 
 def test_kernel_timing_modes(system):
-    # block mode: one event pair per graph-replayed block, 4 apply launches per
-    # step; per-launch mode: every apply launch (no graph); results unchanged
+    # block mode: one event pair around the steps of a call, 4 apply launches
+    # per step; per-launch mode: every apply launch (no graph); results unchanged
     G, ugs = system["G"], system["ugs"]
     nt = 50
     _, A, Aave, _ = T.define_afield_lp(nt, DT, *T.pulse_lp_params(0.2, 0.057, 100.0))
@@ -103,5 +103,11 @@ def test_kernel_timing_modes(system):
     for e, j, _, _ in res[1:]:
         np.testing.assert_allclose(e, res[0][0], rtol=0, atol=1e-12)
         np.testing.assert_allclose(j, res[0][1], rtol=0, atol=1e-14)
+    # fewer steps than one captured graph chunk (25): still one timed interval
+    G.upload(ugs)
+    G.set_kernel_timing(True, block=True)
+    G.propagate(A[:10], Aave[:10])
+    assert G.kernel_stats()[0] == 40
+    G.set_kernel_timing(False)
     with pytest.raises(T.TDHFError):
         T._chk(T.load().tdhf_set_kernel_timing(G.h, 3), G.h)
