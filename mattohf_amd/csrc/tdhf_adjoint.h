@@ -76,11 +76,20 @@ __host__ __device__ constexpr int adjoint_slot() {
 // field derivative staging), y, gbar, the exchange sum and K_1 (complex NG
 // each), rho, s, K_H, K_2 (NG doubles each), a small scratch and the chunk-sum
 // scratch.
+// NG = 32 reads the transforms' lane twiddles from an LDS copy of the table
+// (2 NG complex): its geometry (two registers sets of twiddles) would otherwise
+// push the kernel's registers into spills.
+template <int NG>
+__host__ __device__ constexpr bool adjoint_lds_twiddles() {
+  return NG == 32;
+}
+
 template <int NG>
 __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * adjoint_slot<NG>() * sizeof(zd) +
          4 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
-         ((size_t)64 * nw > 4 * NG ? (size_t)64 * nw : 4 * NG) * sizeof(double);
+         ((size_t)64 * nw > 4 * NG ? (size_t)64 * nw : 4 * NG) * sizeof(double) +
+         (adjoint_lds_twiddles<NG>() ? 2 * NG * sizeof(zd) : 0);
 }
 
 template <int NG>
@@ -135,6 +144,8 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   double* sK2 = sKH + NG;
   double* sRed = sK2 + NG;
   double* sPart = sRed + 16;  // max(blockDim.x, 4 NG) doubles (chunk sums)
+  constexpr bool LTW = adjoint_lds_twiddles<NG>();
+  zd* sTw = reinterpret_cast<zd*>(sPart + (64 * nw > 4 * NG ? 64 * nw : 4 * NG));
   static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
                 "slot area too small for the epilogue scratch");
 
@@ -153,7 +164,20 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   }
   const double kxv = gload(a.kx + ik);
   F fft;
-  fft.init(a.tw, l);
+  if constexpr (LTW) {
+    for (int i = tid; i < 2 * NG; i += blockDim.x) sTw[i] = gload(a.tw + i);
+  } else {
+    fft.init(a.tw, l);
+  }
+  // lane twiddles: tw[l R + g0] (forward), tw[NG + l R + i] (inverse, Q > 1)
+  auto fwf = [&](int g0) { return sTw[l * R + g0]; };
+  auto iwf = [&](int i) { return sTw[NG + l * R + i]; };
+  auto fwd = [&](zd (&v)[R]) {
+    if constexpr (LTW) fft.fwd_t(v, T, l, fwf); else fft.fwd(v, T, l);
+  };
+  auto inv = [&](zd (&v)[R]) {
+    if constexpr (LTW) fft.inv_t(v, T, l, fwf, iwf); else fft.inv(v, T, l);
+  };
   const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = L*m + l, L even
   const bool out_occ = ib < a.nbocc;
 
@@ -197,7 +221,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
         constexpr int m = decltype(mc)::value;
         v[m] = zmulc(sY[L * m + l], uq[m]);
       });
-      fft.fwd(v, T, l);
+      fwd(v);
       // rho_hat(NG/2) sits in lane 0 (of the group), register JNYQ
       const double nx = __shfl(v[JNYQ].x, gw * L, 64);
       const double ny = __shfl(v[JNYQ].y, gw * L, 64);
@@ -205,7 +229,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
         constexpr int j = decltype(jc)::value;
         v[j] = zscal(v[j], vtr[j]);
       });
-      fft.inv(v, T, l);
+      inv(v);
       const zd fix = mk(sgn * dnyq * nx, sgn * dnyq * ny);
       static_for<0, R>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
@@ -237,12 +261,12 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
         cx += __shfl_xor(cx, off);
         cy += __shfl_xor(cy, off);
       }
-      fft.fwd(v, T, l);
+      fwd(v);
       static_for<0, R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         v[j] = zscal(v[j], vtr[j]);
       });
-      fft.inv(v, T, l);
+      inv(v);
       const zd fix = mk(sgn * dnyq * cx, sgn * dnyq * cy);
       static_for<0, R>([&](auto mc) {
         constexpr int m = decltype(mc)::value;

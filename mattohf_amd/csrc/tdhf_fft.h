@@ -222,13 +222,19 @@ struct GroupFFT {
     return mk(cs, sn);
   }
 
-  // spatial layout -> G layout (forward, exp(-i...)), T = this group's tile
+  // spatial layout -> G layout (forward, exp(-i...)), T = this group's tile;
+  // the lane twiddles come from registers (fwd) or from an accessor (fwd_t:
+  // fwf(g0) = fw[g0] of this lane, e.g. read from an LDS copy of the table)
   __device__ __forceinline__ void fwd(zd (&v)[R], zd* T, int l) const {
+    fwd_t(v, T, l, [&](int g0) { return fw[g0]; });
+  }
+  template <class FW>
+  __device__ __forceinline__ void fwd_t(zd (&v)[R], zd* T, int l, FW&& fwf) const {
     zd y[R];
     dft<R, -1, 1, 0, R>(v, y);
     static_for<1, R>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
-      y[g0] = zmul(y[g0], fw[g0]);
+      y[g0] = zmul(y[g0], fwf(g0));
     });
     static_for<0, R>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
@@ -253,8 +259,14 @@ struct GroupFFT {
     });
   }
 
-  // G layout -> spatial layout (backward, exp(+i...))
+  // G layout -> spatial layout (backward, exp(+i...)); inv_t as fwd_t, with
+  // iwf(i) = iw[i] for Q > 1
   __device__ __forceinline__ void inv(zd (&v)[R], zd* T, int l) const {
+    inv_t(v, T, l, [&](int g0) { return fw[g0]; },
+          [&](int i) { return iw[(Q == 1) ? 0 : i]; });
+  }
+  template <class FW, class IW>
+  __device__ __forceinline__ void inv_t(zd (&v)[R], zd* T, int l, FW&& fwf, IW&& iwf) const {
     zd z[R];
     static_for<0, Q>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
@@ -268,12 +280,12 @@ struct GroupFFT {
     if constexpr (Q == 1) {
       static_for<1, R>([&](auto xc) {
         constexpr int x0 = decltype(xc)::value;
-        z[x0] = zmulc(z[x0], fw[x0]);
+        z[x0] = zmulc(z[x0], fwf(x0));
       });
     } else {
       static_for<0, R>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
-        z[i] = zmul(z[i], iw[i]);
+        z[i] = zmul(z[i], iwf(i));
       });
     }
     static_for<0, R>([&](auto ic) {
