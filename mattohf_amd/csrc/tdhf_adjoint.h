@@ -76,9 +76,9 @@ __host__ __device__ constexpr int adjoint_slot() {
 // field derivative staging), y, gbar, the exchange sum and K_1 (complex NG
 // each), rho, s, K_H, K_2 (NG doubles each), a small scratch and the chunk-sum
 // scratch.
-// NG = 32 reads the transforms' lane twiddles from an LDS copy of the table
-// (2 NG complex): its geometry (two registers sets of twiddles) would otherwise
-// push the kernel's registers into spills.
+// NG = 32 reads the transforms' lane twiddles from an LDS copy of the table:
+// its geometry (two register sets of twiddles) would otherwise push the
+// kernel's registers into spills.
 template <int NG>
 __host__ __device__ constexpr bool adjoint_lds_twiddles() {
   return NG == 32;
@@ -89,7 +89,7 @@ __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
   return (size_t)nw * (64 / GroupFFT<NG>::L) * adjoint_slot<NG>() * sizeof(zd) +
          4 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
          ((size_t)64 * nw > 4 * NG ? (size_t)64 * nw : 4 * NG) * sizeof(double) +
-         (adjoint_lds_twiddles<NG>() ? 2 * NG * sizeof(zd) : 0);
+         (adjoint_lds_twiddles<NG>() ? GroupFFT<NG>::TWN * sizeof(zd) : 0);
 }
 
 template <int NG>
@@ -165,18 +165,15 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   const double kxv = gload(a.kx + ik);
   F fft;
   if constexpr (LTW) {
-    for (int i = tid; i < 2 * NG; i += blockDim.x) sTw[i] = gload(a.tw + i);
+    for (int i = tid; i < F::TWN; i += blockDim.x) sTw[i] = gload(a.tw + i);
   } else {
     fft.init(a.tw, l);
   }
-  // lane twiddles: tw[l R + g0] (forward), tw[NG + l R + i] (inverse, Q > 1)
-  auto fwf = [&](int g0) { return sTw[l * R + g0]; };
-  auto iwf = [&](int i) { return sTw[NG + l * R + i]; };
   auto fwd = [&](zd (&v)[R]) {
-    if constexpr (LTW) fft.fwd_t(v, T, l, fwf); else fft.fwd(v, T, l);
+    if constexpr (LTW) fft.fwd_lds(v, T, l, sTw); else fft.fwd(v, T, l);
   };
   auto inv = [&](zd (&v)[R]) {
-    if constexpr (LTW) fft.inv_t(v, T, l, fwf, iwf); else fft.inv(v, T, l);
+    if constexpr (LTW) fft.inv_lds(v, T, l, sTw); else fft.inv(v, T, l);
   };
   const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = L*m + l, L even
   const bool out_occ = ib < a.nbocc;

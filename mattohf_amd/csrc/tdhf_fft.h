@@ -222,11 +222,17 @@ struct GroupFFT {
     return mk(cs, sn);
   }
 
+  static constexpr int TWN = 2 * NG;  // entries of the twiddle table
+
   // spatial layout -> G layout (forward, exp(-i...)), T = this group's tile;
-  // the lane twiddles come from registers (fwd) or from an accessor (fwd_t:
-  // fwf(g0) = fw[g0] of this lane, e.g. read from an LDS copy of the table)
+  // the lane twiddles come from registers (fwd, after init), from a copy of
+  // the twiddle table (fwd_lds, e.g. in LDS: no registers held) or from an
+  // accessor (fwd_t: fwf(g0) = fw[g0] of this lane)
   __device__ __forceinline__ void fwd(zd (&v)[R], zd* T, int l) const {
     fwd_t(v, T, l, [&](int g0) { return fw[g0]; });
+  }
+  __device__ __forceinline__ void fwd_lds(zd (&v)[R], zd* T, int l, const zd* tw) const {
+    fwd_t(v, T, l, [&](int g0) { return tw[l * R + g0]; });
   }
   template <class FW>
   __device__ __forceinline__ void fwd_t(zd (&v)[R], zd* T, int l, FW&& fwf) const {
@@ -264,6 +270,10 @@ struct GroupFFT {
   __device__ __forceinline__ void inv(zd (&v)[R], zd* T, int l) const {
     inv_t(v, T, l, [&](int g0) { return fw[g0]; },
           [&](int i) { return iw[(Q == 1) ? 0 : i]; });
+  }
+  __device__ __forceinline__ void inv_lds(zd (&v)[R], zd* T, int l, const zd* tw) const {
+    inv_t(v, T, l, [&](int g0) { return tw[l * R + g0]; },
+          [&](int i) { return tw[NG + l * R + i]; });
   }
   template <class FW, class IW>
   __device__ __forceinline__ void inv_t(zd (&v)[R], zd* T, int l, FW&& fwf, IW&& iwf) const {
@@ -362,13 +372,34 @@ struct GroupFFT<128> {
     return 44 * a + 19 * s + 5 * h1 + l1;
   }
 
+  static constexpr int TWN = 144;  // entries of the twiddle table
+
+  // fwd / inv with the twiddles in registers (after init), fwd_lds / inv_lds
+  // with them read from a copy of the table (e.g. in LDS), fwd_t / inv_t with
+  // accessors fwf(g0) = fw[g0], fqf(h) = fq[h] of this lane
   __device__ __forceinline__ void fwd(zd (&v)[8], zd* T, int l) const {
+    fwd_t(v, T, l, [&](int g0) { return fw[g0]; }, [&](int h) { return fq[h]; });
+  }
+  __device__ __forceinline__ void fwd_lds(zd (&v)[8], zd* T, int l, const zd* tw) const {
+    fwd_t(v, T, l, [&](int g0) { return tw[l * 8 + g0]; },
+          [&](int h) { return tw[128 + (l & 3) * 4 + h]; });
+  }
+  __device__ __forceinline__ void inv(zd (&v)[8], zd* T, int l) const {
+    inv_t(v, T, l, [&](int g0) { return fw[g0]; }, [&](int h) { return fq[h]; });
+  }
+  __device__ __forceinline__ void inv_lds(zd (&v)[8], zd* T, int l, const zd* tw) const {
+    inv_t(v, T, l, [&](int g0) { return tw[l * 8 + g0]; },
+          [&](int h) { return tw[128 + (l & 3) * 4 + h]; });
+  }
+
+  template <class FW, class FQ>
+  __device__ __forceinline__ void fwd_t(zd (&v)[8], zd* T, int l, FW&& fwf, FQ&& fqf) const {
     const int a = l >> 2, l1 = l & 3;
     zd y[8];
     dft<8, -1, 1, 0, 8>(v, y);
     static_for<1, 8>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
-      y[g0] = zmul(y[g0], fw[g0]);
+      y[g0] = zmul(y[g0], fwf(g0));
     });
     static_for<0, 8>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
@@ -387,7 +418,7 @@ struct GroupFFT<128> {
       dft<4, -1, 1, s * 4, 8>(z, t);
       static_for<0, 4>([&](auto hc) {
         constexpr int h1 = decltype(hc)::value;
-        T[t2(a, s, h1, l1)] = h1 == 0 ? t[0] : zmul(t[h1], fq[h1]);
+        T[t2(a, s, h1, l1)] = h1 == 0 ? t[0] : zmul(t[h1], fqf(h1));
       });
     });
     wave_sync();
@@ -407,7 +438,8 @@ struct GroupFFT<128> {
     });
   }
 
-  __device__ __forceinline__ void inv(zd (&v)[8], zd* T, int l) const {
+  template <class FW, class FQ>
+  __device__ __forceinline__ void inv_t(zd (&v)[8], zd* T, int l, FW&& fwf, FQ&& fqf) const {
     const int a = l >> 2, c = l & 3;
     static_for<0, 2>([&](auto sc) {
       constexpr int s = decltype(sc)::value;
@@ -415,7 +447,7 @@ struct GroupFFT<128> {
       dft<4, 1, 1, s * 4, 8>(v, t);  // over h2 -> l1
       static_for<0, 4>([&](auto kc) {
         constexpr int k1 = decltype(kc)::value;
-        T[t2(a, s, c, k1)] = k1 == 0 ? t[0] : zmulc(t[k1], fq[k1]);  // w16^(-c l1)
+        T[t2(a, s, c, k1)] = k1 == 0 ? t[0] : zmulc(t[k1], fqf(k1));  // w16^(-c l1)
       });
     });
     wave_sync();
@@ -438,7 +470,7 @@ struct GroupFFT<128> {
     zd y[8];
     static_for<0, 8>([&](auto gc) {
       constexpr int g0 = decltype(gc)::value;
-      y[g0] = g0 == 0 ? T[t1(0, l)] : zmulc(T[t1(g0, l)], fw[g0]);
+      y[g0] = g0 == 0 ? T[t1(0, l)] : zmulc(T[t1(g0, l)], fwf(g0));
     });
     wave_sync();
     dft<8, 1, 1, 0, 8>(y, v);
