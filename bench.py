@@ -1,6 +1,6 @@
 """Benchmark of the plain FP64 TDHF RK4 path on one MI355X.
 
-Workload (one trajectory, one launch sequence per step): 1D periodic cell
+Workload (one trajectory, 4 fused launches per RK4 step): 1D periodic cell
 a=10 bohr, NG=64, NK=64, NB=NBocc=4 (Z=8, alpha=1), soft-Coulomb e-e and
 e-ion, RK4 with E(t), N(t), J(t) evaluated after every step, sin^2 pulse
 A0=0.3, w=0.057 Ha, T=400 au.  dt = 0.01 au: the specified 0.02 au exceeds the
@@ -9,12 +9,17 @@ per-step cost does not depend on dt.  Orbitals start from a ground state
 computed on the GPU (not timed).
 
 A step = one RK4 step of the trajectory (4 fused H-apply/stage launches) plus
-the observables.  value = steps / wall seconds (traj-steps/s) with the state
-resident in HBM.  The roofline entry uses the algorithmic FP64 flop count
-per launch (F_step/4, with F_step from the reference algorithm's formula)
-over the average duration of the fused apply kernel, measured with HIP events
-on the stream it runs on.  The CPU baseline is the FP64 C oracle (same
-algorithm, OpenMP over output k-points) on a bounded sample of steps.
+the observables.  value = K / wall seconds (traj-steps/s) of one
+tdhf_rt_advance(K) call, with the state and the field resident in HBM (the
+call returns after the device finished; graph-replayed chunks).  The roofline
+entry uses the algorithmic FP64 flop count per launch (F_step/4, with F_step
+from the reference algorithm's formula) over the average duration of the
+fused apply kernel, measured with HIP events on the stream it runs on.  The
+whole pulse (40000 steps) is then run once; the FP64 C oracle (same
+algorithm, OpenMP over output k-points) takes the device state at mid-pulse,
+runs a bounded sample of steps (the CPU baseline), and its E and J are
+compared with the device's (parity entry).  The batched and adjoint legs run
+afterwards, each in a child process, and are merged into the same line.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -69,36 +74,61 @@ def setup(cfg, gs_iters=60):
     return G, vion, u, eh
 
 
-def cpu_baseline(cfg, vion, G, u, A, Aave, budget_s=12.0, max_steps=2000):
+def host_threads():
+    """Host threads for the CPU legs: the CPU set this process may run on,
+    capped by OMP_NUM_THREADS when that is set (the GPU box allots each GPU a
+    share of its host cores and sets OMP_NUM_THREADS to it)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def oracle_for(cfg, vion, vq):
+    """The CPU oracle of the benchmark cell, fed with the exchange table rows
+    the device built (vq[d], d = ikr - ikc + NK - 1; the tables themselves are
+    compared with the oracle's in tests/test_gpu_parity.py)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
 
-    threads = min(os.cpu_count() or 1, 16)
-    po.set_threads(threads)
     ng, nk, nb, nbo, a = cfg["ng"], cfg["nk"], cfg["nb"], cfg["nbocc"], cfg["acell"]
     S = po.System(ng, nk, nb, nbo, a)
-    vcg, vq = G.tables()
     full = np.empty((nk * nk, ng))
     for ikr in range(nk):
         for ikc in range(nk):
             full[ikr * nk + ikc] = vq[ikr - ikc + nk - 1]
-    O = po.Oracle(S, vion, full)
-    uu = u.copy(order="F")
-    n = 0
+    return po, po.Oracle(S, vion, full)
+
+
+def cpu_baseline(cfg, vion, vq, u_start, A, Aave, s0, budget_s=12.0):
+    """The FP64 C oracle (same algorithm: RK4 with E, N, J after every step,
+    src/RT_calculation.f90:38-47; OpenMP over output k-points) on a bounded
+    sample of steps s0, s0+1, ... of the benchmark trajectory, started from
+    the device state after step s0-1.  Returns (cpu_baseline entry, oracle E,
+    oracle J) of the sample."""
+    threads = host_threads()
+    po, O = oracle_for(cfg, vion, vq)
+    po.set_threads(threads)
+    uu = np.asfortranarray(u_start).copy(order="F")
+    es, js = [], []
     t0 = time.perf_counter()
-    while n < max_steps:
-        # one step of RT_calculation: RK4 + N, E, J (src/RT_calculation.f90:42-47)
-        uu = O.RT_RK4_ubk(uu, cfg["dt"], Aave[n], A[n])
+    n = 0
+    while s0 + n < len(A):
+        it = s0 + n
+        uu = O.RT_RK4_ubk(uu, cfg["dt"], Aave[it], A[it])
         O.nele(uu)
-        O.ubk_Etot(uu, A[n])
-        O.ubk_Je(uu, A[n])
+        es.append(O.ubk_Etot(uu, A[it]))
+        js.append(O.ubk_Je(uu, A[it]))
         n += 1
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "traj-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n} RK4 steps (+E/N/J each step) of the same trajectory, FP64 C oracle, "
-                      f"OpenMP {threads} threads, {dt:.1f} s"}
+    entry = {"value": n / dt, "unit": "traj-steps/s", "cores": threads, "kind": "port",
+             "sample": f"{n} RK4 steps (+E/N/J each step) of the benchmark trajectory from step "
+                       f"{s0} (mid-pulse), FP64 C oracle (same algorithm), OpenMP {threads} "
+                       f"threads, {dt:.1f} s"}
+    return entry, np.array(es), np.array(js)
 
 
 def copy_probe(nbytes=1 << 30, reps=4):
@@ -254,7 +284,7 @@ def adjoint_bench(seed=4, cpu_check=True):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
 
-        po.set_threads(min(os.cpu_count() or 1, 16))
+        po.set_threads(host_threads())
         ns = c["cpu_steps"]
         S = po.System(ng, nk, nb, c["nbocc"], a)
         vcg, vq = G.tables()
@@ -355,86 +385,74 @@ def adjoint_scale_bench(nt=8, seed=20261205, gs_iters=40):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--timing-steps", type=int, default=100)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--batch-traj", type=int, default=256)
-    ap.add_argument("--batch-steps", type=int, default=30)
-    ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--no-adjoint-scale", action="store_true")
-    ap.add_argument("--no-full-run", action="store_true",
-                    help="skip the whole-pulse plain run (T/dt steps in one call)")
-    ap.add_argument("--adjoint-scale-steps", type=int, default=8)
-    ap.add_argument("--only", choices=["adjoint-scale", "adjoint"], default=None,
-                    help="run one secondary leg alone and print its JSON")
-    ap.add_argument("--gs-iters", type=int, default=60,
-                    help="ground-state iterations before the timed run (0: orthonormalised "
-                         "initial guess; used for profiling runs)")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="run only a plain device copy of known size (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.copy_probe:
-        copy_probe()
-        return
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the path has no multi-GPU mode); "
-              "use --gpus 1", file=sys.stderr)
-        sys.exit(2)
+LEGS = ("batched", "adjoint", "adjoint-scale")
 
+
+def run_leg(leg, args, timeout_s):
+    """One secondary leg in a child process (its failure cannot take the
+    headline line with it); returns its JSON object or an error entry."""
+    import subprocess
+
+    cmd = [sys.executable, os.path.abspath(__file__), "--only", leg,
+           "--adjoint-scale-steps", str(args.adjoint_scale_steps),
+           "--batch-traj", str(args.batch_traj), "--batch-steps", str(args.batch_steps)]
+    if args.no_cpu:
+        cmd.append("--no-cpu")
+    t0 = time.perf_counter()
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s} s"}
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"exit {r.returncode}", "stderr_tail": r.stderr[-800:]}
+    out = json.loads(lines[-1])
+    out["leg_wall_s"] = time.perf_counter() - t0
+    return out
+
+
+def headline(args):
+    """The plain-path measurement, its kernel timing, the whole-pulse run with
+    its mid-pulse parity sample against the oracle, and the CPU baseline."""
     from mattohf_amd import tdhf as T
 
-    if args.only == "adjoint-scale":
-        print(json.dumps({"adjoint_scale": adjoint_scale_bench(nt=args.adjoint_scale_steps)}),
-              flush=True)
-        return
-    if args.only == "adjoint":
-        print(json.dumps({"adjoint": adjoint_bench(cpu_check=not args.no_cpu)}), flush=True)
-        return
     cfg = dict(CFG)
     G, vion, u, eh = setup(cfg, gs_iters=args.gs_iters)
-    nt = args.warmup + args.steps + args.timing_steps
-    _, A, Aave, _ = T.define_afield_lp(nt, cfg["dt"], *T.pulse_lp_params(cfg["A0"], cfg["omega"],
-                                                                          cfg["T"]))
+    W, K, ts, te = args.warmup, args.steps, args.timing_steps, 25
+    nfull = int(round(cfg["T"] / cfg["dt"]))
+    lp = T.pulse_lp_params(cfg["A0"], cfg["omega"], cfg["T"])
+    nrun = W + K + ts + te
+    _, A, Aave, _ = T.define_afield_lp(max(nrun, nfull), cfg["dt"], *lp)
+
+    # timed region: K steps of the resident state with the field resident too
     G.upload(u)
-    W, K = args.warmup, args.steps
-    if W:
-        G.propagate(A[:W], Aave[:W])
+    G.rt_begin(A[:nrun], Aave[:nrun])
+    G.rt_advance(W)  # warm-up; the first advance captures the step graphs
     t0 = time.perf_counter()
-    e, n, j, _ = G.propagate(A[W:W + K], Aave[W:W + K])
-    t1 = time.perf_counter()
-    wall = t1 - t0
+    G.rt_advance(K)  # synchronous: returns after the device finished
+    wall = time.perf_counter() - t0
     value = K / wall
 
-    # duration of the fused apply kernel: HIP events on its stream around the
-    # graph-replayed steps of one call (4 launches per step, no per-launch events)
+    # duration of the fused apply kernel: HIP events on its stream around ts
+    # graph-replayed steps (4 launches per step, no per-launch events)
     G.set_kernel_timing(True, block=True)
-    ts = args.timing_steps
-    G.propagate(A[W + K:W + K + ts], Aave[W + K:W + K + ts])
+    G.rt_advance(ts)
     launches, total_ms = G.kernel_stats()
-    G.set_kernel_timing(False)
-    if launches < 1:
-        raise SystemExit("bench.py: no timed apply launches (--timing-steps must be >= 1)")
-    avg_s = total_ms / launches / 1e3
     # the same with an event pair around every launch (graph replay off): an
     # upper bound, each pair adds its own overhead to the interval
     G.set_kernel_timing(True)
-    te = min(25, ts)
-    G.propagate(A[W + K:W + K + te], Aave[W + K:W + K + te])
+    G.rt_advance(te)
     launches_ev, total_ms_ev = G.kernel_stats()
     G.set_kernel_timing(False)
-    f_launch = flops_per_step(cfg["ng"], cfg["nk"], cfg["nb"], cfg["nbocc"]) / 4.0
+    e, n, j, i3 = G.rt_end()
+    if launches < 1:
+        raise SystemExit("bench.py: no timed apply launches (--timing-steps must be >= 1)")
+    avg_s = total_ms / launches / 1e3
+    f_step = flops_per_step(cfg["ng"], cfg["nk"], cfg["nb"], cfg["nbocc"])
+    f_launch = f_step / 4.0
     achieved = f_launch / avg_s / 1e12
-
-    finite = bool(np.isfinite(e).all() and np.isfinite(j).all())
-    # the FP64 rate this device sustains on independent FMAs (its clock under
-    # FP64 load is below the one the published peak assumes)
     fma_tflops, _ = T.fp64_fma_probe()
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -449,17 +467,21 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Init_wf + GPU ground state; sin^2 pulse)",
         "config": {
-            "workload": "plain FP64 TDHF RK4, one trajectory per launch sequence, E/N/J every step",
+            "workload": "plain FP64 TDHF RK4, one trajectory, 4 fused launches per step, "
+                        "E/N/J every step; a=10 NG=64 NK=64 NB=NBocc=4 Z=8",
             "acell": cfg["acell"], "NG": cfg["ng"], "NK": cfg["nk"], "NB": cfg["nb"],
             "NBocc": cfg["nbocc"], "Z": cfg["z"], "dt": cfg["dt"],
             "dt_note": "0.02 au exceeds the RK4 stability bound of NG=64, a=10",
             "pulse": {"A0": cfg["A0"], "omega": cfg["omega"], "T": cfg["T"]},
-            "flop_per_traj_step": flops_per_step(cfg["ng"], cfg["nk"], cfg["nb"], cfg["nbocc"]),
-            "observables_finite": finite,
+            "flop_per_traj_step": f_step,
+            "timed_call": "tdhf_rt_advance(K) on the device-resident state and field, "
+                          "graph-replayed chunks",
+            "observables_finite": bool(np.isfinite(e).all() and np.isfinite(j).all()),
         },
         "roofline": {
-            "bound": "mfma",
-            "bound_detail": "FP64 vector ALU (no dense contraction on the path); peak = MI355X FP64",
+            "bound": "fp64-valu",
+            "bound_detail": "FP64 vector ALU (no dense contraction on the path, no MFMA); "
+                            "peak = MI355X FP64 vector",
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
@@ -473,59 +495,130 @@ def main():
             "kernel": "apply_kernel<64> (fused exchange + H + RK4 stage)",
             "avg_launch_us": avg_s * 1e6,
             "avg_launch_us_method": "HIP events on the kernel's stream around the "
-                                    "graph-replayed steps of one call, 4 launches per step",
-            "avg_launch_us_per_launch_events": total_ms_ev / launches_ev * 1e3,
+                                    f"{ts} graph-replayed steps of one call, 4 launches per step",
+            "avg_launch_us_per_launch_events": total_ms_ev / max(launches_ev, 1) * 1e3,
             "flop_per_launch": f_launch,
             "timed_launches": launches,
         },
         "cpu_baseline": None,
     }
-    if not args.no_full_run:
-        # the whole pulse of the benchmark cell in one call (T = 400 au: 40000 steps
-        # at dt = 0.01), from the ground state: norm and energy bookkeeping
-        nfull = int(round(cfg["T"] / cfg["dt"]))
-        _, Af, Aavef, _ = T.define_afield_lp(nfull, cfg["dt"], *T.pulse_lp_params(
-            cfg["A0"], cfg["omega"], cfg["T"]))
-        G.upload(u)
-        t0 = time.perf_counter()
-        ef, nf, jf, i3 = G.propagate(Af, Aavef)
-        tw = time.perf_counter() - t0
-        out["full_run"] = {
-            "steps": nfull,
-            "duration_au": nfull * cfg["dt"],
-            "wall_s": tw,
-            "traj_steps_per_s": nfull / tw,
-            "N_rel_drift_max": float(np.abs(nf - i3[1]).max() / i3[1]),
-            "N_note": "RK4 (RT_RK4_ubk) is not unitary: this is the reference algorithm's "
-                      "own norm change, reproduced to 1e-10 by the parity tests",
-            "E_absorbed_Ha": float(ef[-1] - i3[0]),
-            "J_max_abs": float(np.abs(jf).max()),
-            "observables_finite": bool(np.isfinite(ef).all() and np.isfinite(jf).all()),
-        }
+
+    # the whole pulse (T = 400 au: 40000 steps at dt = 0.01) from the ground
+    # state in one run, with the state taken at mid-pulse for the parity sample
+    s0 = nfull // 2
+    G.upload(u)
+    G.rt_begin(A[:nfull], Aave[:nfull])
+    t0 = time.perf_counter()
+    G.rt_advance(s0)
+    t1 = time.perf_counter()
+    u_mid = G.download()
+    t2 = time.perf_counter()
+    G.rt_advance(nfull - s0)
+    t3 = time.perf_counter()
+    ef, nf, jf, i3f = G.rt_end()
+    vcg, vq = G.tables()
     G.close()
-    if not args.no_batch:
-        rng = np.random.default_rng(20261202)
-        mixed = [int(k) for k in rng.choice([16, 32, 64], size=args.batch_traj)]
-        bm = batch_bench(cfg, mixed, args.batch_steps, 5, seed=20261202)
-        bu = batch_bench(cfg, [64] * max(1, args.batch_traj // 4), args.batch_steps, 5,
-                         seed=20261203)
-        out["batched"] = {
-            "workload": "independent trajectories, one launch per RK4 stage for all, "
-                        "hipGraph-replayed steps, autotuned waves/workgroup",
-            "seed": 20261202,
-            "mixed_nk": bm,
-            "uniform_nk64": bu,
-            "batched_vs_plain_at_nk64": bu["traj_steps_per_s"] / value,
-        }
-    if not args.no_adjoint:
-        out["adjoint"] = adjoint_bench(cpu_check=not args.no_cpu)
-    if not args.no_adjoint_scale:
-        out["adjoint_scale"] = adjoint_scale_bench(nt=args.adjoint_scale_steps)
+    tw = (t1 - t0) + (t3 - t2)
+    out["full_run"] = {
+        "steps": nfull,
+        "duration_au": nfull * cfg["dt"],
+        "wall_s": tw,
+        "traj_steps_per_s": nfull / tw,
+        "N_rel_drift_max": float(np.abs(nf - i3f[1]).max() / i3f[1]),
+        "N_note": "RK4 (RT_RK4_ubk) is not unitary: this is the reference algorithm's "
+                  "own norm change",
+        "E_absorbed_Ha": float(ef[-1] - i3f[0]),
+        "J_max_abs": float(np.abs(jf).max()),
+        "observables_finite": bool(np.isfinite(ef).all() and np.isfinite(jf).all()),
+    }
     if not args.no_cpu:
-        G2 = setup(cfg, gs_iters=0)[0]
-        out["cpu_baseline"] = cpu_baseline(cfg, vion, G2, u, A, Aave, budget_s=args.cpu_seconds)
-        G2.close()
+        entry, eo, jo = cpu_baseline(cfg, vion, vq, u_mid, A[:nfull], Aave[:nfull], s0,
+                                     budget_s=args.cpu_seconds)
+        out["cpu_baseline"] = entry
+        m = len(eo)
+        dj = float(np.abs(jf[s0:s0 + m] - jo).max() / np.abs(jo).max())
+        de = float(np.abs(ef[s0:s0 + m] - eo).max())
+        out["parity"] = {
+            "steps": m, "from_step": s0,
+            "rel_dJ": dj, "abs_dE_Ha": de,
+            "tolerance": {"rel_dJ": 1e-8, "abs_dE_Ha": 1e-10},
+            "ok": bool(dj < 1e-8 and de < 1e-10),
+            "method": "device observables of the whole-pulse run vs the FP64 oracle from the "
+                      "same mid-pulse state, same field; rel_dJ = max|dJ|/max|J|",
+        }
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--timing-steps", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--batch-traj", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=30)
+    ap.add_argument("--no-legs", action="store_true",
+                    help="headline only (no batched / adjoint legs)")
+    ap.add_argument("--adjoint-scale-steps", type=int, default=8)
+    ap.add_argument("--leg-timeout", type=float, default=300.0)
+    ap.add_argument("--only", choices=list(LEGS), default=None,
+                    help="run one secondary leg alone and print its JSON")
+    ap.add_argument("--gs-iters", type=int, default=60,
+                    help="ground-state iterations before the timed run (0: orthonormalised "
+                         "initial guess; used for profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain device copy of known size (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
+    if args.gpus != 1:
+        print("bench.py: every GPU box of this build has one GPU and the path has no "
+              "multi-GPU mode; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    if args.only == "adjoint-scale":
+        print(json.dumps({"adjoint_scale": adjoint_scale_bench(nt=args.adjoint_scale_steps)}),
+              flush=True)
+        return
+    if args.only == "adjoint":
+        print(json.dumps({"adjoint": adjoint_bench(cpu_check=not args.no_cpu)}), flush=True)
+        return
+    if args.only == "batched":
+        print(json.dumps({"batched": batched_leg(args)}), flush=True)
+        return
+
+    out = headline(args)
+    print(f"bench: headline {out['value']:.1f} traj-steps/s, "
+          f"{out['roofline']['avg_launch_us']:.2f} us/launch", file=sys.stderr, flush=True)
+    if not args.no_legs:
+        for leg in LEGS:
+            r = run_leg(leg, args, args.leg_timeout)
+            key = leg.replace("-", "_")
+            out[key] = r.get(key, r)
+            print(f"bench: leg {leg} done", file=sys.stderr, flush=True)
+        b = out.get("batched", {})
+        if "uniform_nk64" in b:
+            b["batched_vs_plain_at_nk64"] = b["uniform_nk64"]["traj_steps_per_s"] / out["value"]
     print(json.dumps(out), flush=True)
+
+
+def batched_leg(args):
+    cfg = dict(CFG)
+    rng = np.random.default_rng(20261202)
+    mixed = [int(k) for k in rng.choice([16, 32, 64], size=args.batch_traj)]
+    bm = batch_bench(cfg, mixed, args.batch_steps, 5, seed=20261202)
+    bu = batch_bench(cfg, [64] * max(1, args.batch_traj // 4), args.batch_steps, 5,
+                     seed=20261203)
+    return {
+        "workload": "independent trajectories, one launch per RK4 stage for all, "
+                    "hipGraph-replayed steps, autotuned waves/workgroup",
+        "seed": 20261202,
+        "mixed_nk": bm,
+        "uniform_nk64": bu,
+    }
 
 
 if __name__ == "__main__":

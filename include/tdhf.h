@@ -116,6 +116,24 @@ int tdhf_state_download(tdhf_ctx* ctx, double* u);
 int tdhf_rt_propagate(tdhf_ctx* ctx, int nt, const double* A, const double* Aave, double* etot,
                       double* nele, double* jtot, double* init3);
 
+/* The same loop split into calls, for callers that keep the field on the
+ * device and advance the resident state in pieces (the time loop of
+ * RT_calculation, src/RT_calculation.f90:38-60, with its field arrays from
+ * Define_Afield, :175-244, uploaded once).
+ *   tdhf_rt_begin   : uploads A, Aave (nt values each, as tdhf_rt_run) and
+ *                     sets the step counter to 0; the state is the one last
+ *                     uploaded (tdhf_state_upload) or propagated
+ *   tdhf_rt_advance : the next n steps (counting from the begin), in place
+ *                     on the device state; fails if they exceed nt
+ *   tdhf_rt_end     : observables of every step advanced since the begin
+ *                     (done values each, as tdhf_rt_run; NULL ok) and init3;
+ *                     closes the run
+ * tdhf_rt_propagate(nt, ...) is begin(nt) + advance(nt) + end.  The state can
+ * be downloaded between advances (tdhf_state_download). */
+int tdhf_rt_begin(tdhf_ctx* ctx, int nt, const double* A, const double* Aave);
+int tdhf_rt_advance(tdhf_ctx* ctx, int n);
+int tdhf_rt_end(tdhf_ctx* ctx, double* etot, double* nele, double* jtot, double* init3);
+
 /* Ground state.  Replaces GS_calculation (src/GS_calculation.f90:26-57) with
  * KS_ubk2_SVD (:124-194), Subspace_diag (:197-237) and Gram_Schmidt
  * (src/mATTOHF.f90:134-157).  Runs nscf iterations (or stops early once

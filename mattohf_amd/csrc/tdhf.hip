@@ -59,10 +59,15 @@ struct tdhf_ctx {
   double *d_A = nullptr, *d_Aave = nullptr;
   long A_cap = 0;
   int* d_step = nullptr;
-  // step graph
-  hipGraphExec_t gexec = nullptr;
-  int graph_steps = 0;
+  // step graphs: one per chunk length of kChunkSteps, captured together the
+  // first time a run needs them (for the field / observables buffers in
+  // graph_key)
+  hipGraphExec_t gexec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  bool graphs_ready = false;
   const void* graph_key[3] = {nullptr, nullptr, nullptr};
+  // run opened by tdhf_rt_begin: field length and steps done so far
+  bool rt_open = false;
+  long rt_nt = 0, rt_done = 0;
   // launch shape of the apply kernel
   int nw = 8;
   size_t lds = 0;
@@ -280,10 +285,18 @@ int ensure_field(tdhf_ctx* c, long n) {
   return 0;
 }
 
+// Chunk lengths of the step graphs: a run of n steps is replayed as the
+// binary decomposition of n over these (plus whole 32-step chunks), so any
+// step count after the first run replays captured graphs only.
+constexpr int kChunkSteps[6] = {32, 16, 8, 4, 2, 1};
+constexpr int kNChunks = 6;
+
 void destroy_graph(tdhf_ctx* c) {
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
-  c->graph_steps = 0;
+  for (auto& g : c->gexec) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  c->graphs_ready = false;
 }
 
 // Enqueues steps [*d_step + j0, *d_step + j0 + n) of the RK4 loop.
@@ -319,12 +332,13 @@ int enqueue_steps(tdhf_ctx* c, int j0, int n) {
   return 0;
 }
 
+// Chunk length of the batched path's step graph.
 constexpr int kGraphSteps = 25;
 
 int run_steps_inner(tdhf_ctx* c, int nt);
 
-// nt RK4 steps; with block timing the whole sequence (graph-replayed chunks and
-// the remainder, 4 apply launches per step) is one timed interval.
+// nt RK4 steps; with block timing the whole sequence (graph-replayed chunks,
+// 4 apply launches per step) is one timed interval.
 int run_steps(tdhf_ctx* c, int nt) {
   if (!c->block_timing || nt < 1) return run_steps_inner(c, nt);
   hipEvent_t b0 = nullptr, b1 = nullptr;
@@ -336,45 +350,58 @@ int run_steps(tdhf_ctx* c, int nt) {
   return 0;
 }
 
-int run_steps_inner(tdhf_ctx* c, int nt) {
-  int done = 0;
-  const bool use_graph = !c->timing && nt >= kGraphSteps;
-  if (use_graph) {
-    const void* key[3] = {c->d_A, c->d_obs, c->d_Aave};
-    if (c->gexec && (c->graph_key[0] != key[0] || c->graph_key[1] != key[1] ||
-                     c->graph_key[2] != key[2]))
+// Captures the graph of every chunk length: kChunkSteps[i] steps followed by
+// the advance of the device step counter.
+int capture_graphs(tdhf_ctx* c) {
+  destroy_graph(c);
+  for (int i = 0; i < kNChunks; ++i) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_steps(c, 0, kChunkSteps[i]);
+    if (rc == 0)
+      hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step,
+                         kChunkSteps[i]);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != 0 || e != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
       destroy_graph(c);
-    if (!c->gexec) {
-      hipGraph_t g = nullptr;
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue_steps(c, 0, kGraphSteps);
-      if (rc == 0)
-        hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step,
-                           kGraphSteps);
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
-      if (rc != 0) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return fail(c, TDHF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-      e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess)
-        return fail(c, TDHF_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
-      c->graph_steps = kGraphSteps;
-      std::memcpy(c->graph_key, key, sizeof key);
+      return rc != 0 ? rc : fail(c, TDHF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
     }
-    while (nt - done >= c->graph_steps) {
-      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
-      done += c->graph_steps;
+    e = hipGraphInstantiate(&c->gexec[i], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      c->gexec[i] = nullptr;
+      destroy_graph(c);
+      return fail(c, TDHF_EDEVICE, "graph instantiate failed: %s", hipGetErrorString(e));
     }
   }
-  if (done < nt) {
-    CHKRET(enqueue_steps(c, 0, nt - done));
-    hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step,
-                       nt - done);
+  c->graphs_ready = true;
+  c->graph_key[0] = c->d_A;
+  c->graph_key[1] = c->d_obs;
+  c->graph_key[2] = c->d_Aave;
+  return 0;
+}
+
+int run_steps_inner(tdhf_ctx* c, int nt) {
+  if (nt < 1) return 0;
+  if (c->timing) {
+    // an event pair around every launch: plain launches
+    CHKRET(enqueue_steps(c, 0, nt));
+    hipLaunchKernelGGL(tdhf::step_advance_kernel, dim3(1), dim3(1), 0, c->stream, c->d_step, nt);
     HIPCHK(c, hipGetLastError());
+    return 0;
   }
+  if (c->graphs_ready && (c->graph_key[0] != (const void*)c->d_A ||
+                          c->graph_key[1] != (const void*)c->d_obs ||
+                          c->graph_key[2] != (const void*)c->d_Aave))
+    destroy_graph(c);
+  if (!c->graphs_ready) CHKRET(capture_graphs(c));
+  int done = 0;
+  for (int i = 0; i < kNChunks; ++i)
+    while (nt - done >= kChunkSteps[i]) {
+      HIPCHK(c, hipGraphLaunch(c->gexec[i], c->stream));
+      done += kChunkSteps[i];
+    }
   return 0;
 }
 
@@ -651,9 +678,13 @@ int tdhf_create(const tdhf_params* p, const double* vion, const double* vcg, con
                      : c->ng == 32 ? (const void*)tdhf::apply_kernel<32>
                      : c->ng == 64 ? (const void*)tdhf::apply_kernel<64>
                                    : (const void*)tdhf::apply_kernel<128>;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds);
+    // the limit is a property of the kernel, shared by every context of the
+    // process: set it to the largest launch any context makes (8 waves), so
+    // a context created later never lowers it below what another one uses
+    const size_t cap = lds_for(c->ng, 8);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
     if (e != hipSuccess && c->lds > 64 * 1024) {
-      fail(c, TDHF_EDEVICE, "cannot raise LDS limit to %zu bytes: %s", c->lds, hipGetErrorString(e));
+      fail(c, TDHF_EDEVICE, "cannot raise LDS limit to %zu bytes: %s", cap, hipGetErrorString(e));
       return bail(TDHF_EDEVICE);
     }
   }
@@ -749,26 +780,52 @@ int tdhf_state_download(tdhf_ctx* c, double* u) {
   return download(c, u, c->d_u0);
 }
 
-int tdhf_rt_propagate(tdhf_ctx* c, int nt, const double* A, const double* Aave, double* etot,
-                      double* nele, double* jtot, double* init3) {
+int tdhf_rt_begin(tdhf_ctx* c, int nt, const double* A, const double* Aave) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
   if (nt < 0 || (nt > 0 && (!A || !Aave))) return fail(c, TDHF_EINVAL, "invalid field arrays");
-  const int rows = c->nb * c->nk;
+  c->rt_open = false;
   CHKRET(ensure_obs(c, (long)nt + 1));
-  if (nt == 0) {
+  if (nt > 0) {
+    CHKRET(ensure_field(c, nt));
+    HIPCHK(c, hipMemcpyAsync(c->d_A, A, nt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_Aave, Aave, nt * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+  }
+  const int zero = 0;
+  HIPCHK(c, hipMemcpyAsync(c->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->rt_open = true;
+  c->rt_nt = nt;
+  c->rt_done = 0;
+  return 0;
+}
+
+int tdhf_rt_advance(tdhf_ctx* c, int n) {
+  if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
+  if (!c->rt_open) return fail(c, TDHF_EINVAL, "tdhf_rt_advance without tdhf_rt_begin");
+  if (n < 0 || c->rt_done + n > c->rt_nt)
+    return fail(c, TDHF_EINVAL, "advance by %d steps: %ld of %ld steps done", n, c->rt_done,
+                c->rt_nt);
+  CHKRET(run_steps(c, n));
+  CHKRET(sync(c));
+  c->rt_done += n;
+  return 0;
+}
+
+int tdhf_rt_end(tdhf_ctx* c, double* etot, double* nele, double* jtot, double* init3) {
+  if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
+  if (!c->rt_open) return fail(c, TDHF_EINVAL, "tdhf_rt_end without tdhf_rt_begin");
+  c->rt_open = false;
+  const long done = c->rt_done;
+  if (done == 0) {
     double r[7];
     CHKRET(observables_of(c, c->d_u0, 0.0, r));
     if (init3) { init3[0] = r[0]; init3[1] = r[1]; init3[2] = r[2]; }
     return 0;
   }
-  CHKRET(ensure_field(c, nt));
-  HIPCHK(c, hipMemcpyAsync(c->d_A, A, nt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->d_Aave, Aave, nt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  const int zero = 0;
-  HIPCHK(c, hipMemcpyAsync(c->d_step, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  CHKRET(run_steps(c, nt));
-  // observables of the final state: step = *d_step - 1 = nt - 1, slot nt
+  const int rows = c->nb * c->nk;
+  // observables of the final state: step = *d_step - 1 = done - 1, slot done
+  // (those of the earlier states came from stage 1 of the next step)
   tdhf::ApplyArgs a = base_args(c);
   a.mode = tdhf::M_OBS;
   a.u = a.w = c->d_u0;
@@ -778,22 +835,29 @@ int tdhf_rt_propagate(tdhf_ctx* c, int nt, const double* A, const double* Aave, 
   a.jstep = -1;
   a.obs = c->d_obs;
   CHKRET(launch_apply(c, a, rows));
-  const int ns = nt + 1;
-  hipLaunchKernelGGL(tdhf::obs_reduce_kernel, dim3((ns + 63) / 64), dim3(64), 0, c->stream,
-                     c->d_obs, c->obs_stride, rows, ns, c->dv, c->vcell, c->d_res);
+  const long ns = done + 1;
+  hipLaunchKernelGGL(tdhf::obs_reduce_kernel, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0,
+                     c->stream, c->d_obs, c->obs_stride, rows, (int)ns, c->dv, c->vcell, c->d_res);
   HIPCHK(c, hipGetLastError());
   std::vector<double> res((size_t)7 * ns);
   HIPCHK(c, hipMemcpyAsync(res.data(), c->d_res, res.size() * sizeof(double),
                            hipMemcpyDeviceToHost, c->stream));
   CHKRET(sync(c));
   if (init3) { init3[0] = res[0]; init3[1] = res[1]; init3[2] = res[2]; }
-  for (int it = 0; it < nt; ++it) {
+  for (long it = 0; it < done; ++it) {
     const double* r = &res[(size_t)7 * (it + 1)];
     if (etot) etot[it] = r[0];
     if (nele) nele[it] = r[1];
     if (jtot) jtot[it] = r[2];
   }
   return 0;
+}
+
+int tdhf_rt_propagate(tdhf_ctx* c, int nt, const double* A, const double* Aave, double* etot,
+                      double* nele, double* jtot, double* init3) {
+  CHKRET(tdhf_rt_begin(c, nt, A, Aave));
+  CHKRET(tdhf_rt_advance(c, nt));
+  return tdhf_rt_end(c, etot, nele, jtot, init3);
 }
 
 int tdhf_rt_run(tdhf_ctx* c, double* u, int nt, const double* A, const double* Aave, double* etot,

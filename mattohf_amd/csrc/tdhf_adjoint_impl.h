@@ -138,8 +138,10 @@ int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
 }
 
 // Raises the LDS limit of the adjoint kernel (outside any stream capture).
+// The limit is shared by every context of the process, so it is set to the
+// largest launch any context makes (the most waves that fit 160 KiB).
 int adj_prepare(tdhf_ctx* c) {
-  int nw = choose_nw(c->ng, c->nk, c->nb);
+  int nw = 8;
   while (nw > (c->ng + 63) / 64 && adj_lds_for(c->ng, nw) > 160 * 1024) --nw;
   const size_t lds = adj_lds_for(c->ng, nw);
   const void* fn = nullptr;

@@ -494,9 +494,13 @@ int tdhf_batch_get_waves(const tdhf_batch* b) { return b ? b->nw : 0; }
 int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
                         int* best_nw) {
   if (!b || nt_trial < 1 || !A || !Aave) return bfail(b, TDHF_EINVAL, "invalid autotune arguments");
-  // keep the state: the trial steps run on a copy
+  // keep the state: the trial steps run on a copy, freed on every return
   zd* save = nullptr;
   CHKRET(balloc(b, &save, b->total));
+  struct FreeOnReturn {
+    zd*& p;
+    ~FreeOnReturn() { dfree(p); }
+  } save_guard{save};
   BHIP(b, hipMemcpyAsync(save, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
   int best = b->nw;
   double best_t = 1e300;
@@ -514,10 +518,7 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
       auto t1 = std::chrono::steady_clock::now();
       dt_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     }
-    if (rc) {
-      dfree(save);
-      return rc;
-    }
+    if (rc) return rc;
     if (dt_ms < best_t) {
       best_t = dt_ms;
       best = nw;
@@ -525,7 +526,6 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
   }
   BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
-  dfree(save);
   b->nw = best;
   tune_cache()[tune_key(b)] = best;
   if (best_nw) *best_nw = best;

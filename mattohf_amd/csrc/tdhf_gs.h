@@ -24,6 +24,7 @@ struct GsWork {
   zd* W = nullptr;   // ncg blocks of (NG,NB,NK)
   zd* HW = nullptr;  // ncg blocks
   double* eps = nullptr;
+  int* nonconv = nullptr;  // set by rr_kernel when its eigensolver does not converge
   int ncg = 0;
 };
 
@@ -67,10 +68,12 @@ __global__ void orth_kernel(zd* W, size_t blk, int ng, int nb, int m, double dv)
 
 // Rayleigh-Ritz at one k-point per workgroup.  m basis columns in W (block
 // stride blk), their H images in HW.  Writes the lowest nb Ritz vectors to
-// uout (NG,NB,NK) and the Ritz values to eig (nb per k).  Dynamic LDS:
-// 2*m*m complex + m ints.
+// uout (NG,NB,NK) and the Ritz values to eig (nb per k).  If the cyclic
+// Jacobi iteration has not converged after 100 sweeps, *nonconv is set to
+// 1 + ik (the caller reports it).  Dynamic LDS: 2*m*m complex + m ints.
 __global__ void __launch_bounds__(256) rr_kernel(const zd* W, const zd* HW, size_t blk, int ng,
-                                                 int nb, int m, double dv, zd* uout, double* eig) {
+                                                 int nb, int m, double dv, zd* uout, double* eig,
+                                                 int* nonconv) {
   extern __shared__ zd sh[];
   zd* A = sh;
   zd* V = sh + (size_t)m * m;
@@ -105,7 +108,8 @@ __global__ void __launch_bounds__(256) rr_kernel(const zd* W, const zd* HW, size
   }
   __syncthreads();
   // cyclic Jacobi (same rotation as the oracle's orc_heev)
-  for (int sweep = 0; sweep < 100; ++sweep) {
+  bool converged = false;
+  for (int sweep = 0; sweep <= 100; ++sweep) {
     double off = 0.0, tot = 0.0;
     for (int e = t; e < m * m; e += nt) {
       const double v = zabs2(A[e]);
@@ -114,7 +118,11 @@ __global__ void __launch_bounds__(256) rr_kernel(const zd* W, const zd* HW, size
     }
     off = bsum(off, red);
     tot = bsum(tot, red);
-    if (off <= 1e-32 * tot) break;
+    if (off <= 1e-32 * tot) {
+      converged = true;
+      break;
+    }
+    if (sweep == 100) break;
     for (int p = 0; p < m - 1; ++p)
       for (int q = p + 1; q < m; ++q) {
         if (t == 0) {
@@ -167,6 +175,7 @@ __global__ void __launch_bounds__(256) rr_kernel(const zd* W, const zd* HW, size
         __syncthreads();
       }
   }
+  if (t == 0 && !converged && nonconv) *nonconv = 1 + ik;
   if (t == 0) {
     for (int i = 0; i < m; ++i) idx[i] = i;
     for (int i = 0; i < m; ++i) {

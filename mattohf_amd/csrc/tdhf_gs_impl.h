@@ -10,8 +10,10 @@ void gs_free(GsWork& g) {
   if (g.W) (void)hipFree(g.W);
   if (g.HW) (void)hipFree(g.HW);
   if (g.eps) (void)hipFree(g.eps);
+  if (g.nonconv) (void)hipFree(g.nonconv);
   g.W = g.HW = nullptr;
   g.eps = nullptr;
+  g.nonconv = nullptr;
   g.ncg = 0;
 }
 
@@ -26,7 +28,26 @@ int gs_alloc(tdhf_ctx* c, int ncg) {
   CHKRET(dalloc(c, &c->gs.W, n));
   CHKRET(dalloc(c, &c->gs.HW, n));
   CHKRET(dalloc(c, &c->gs.eps, (size_t)c->nb * c->nk * 2));
+  CHKRET(dalloc(c, &c->gs.nonconv, 1));
   c->gs.ncg = ncg;
+  return 0;
+}
+
+// Clears the eigensolver's non-convergence flag (before a GS run).
+int gs_flag_clear(tdhf_ctx* c) {
+  HIPCHK(c, hipMemsetAsync(c->gs.nonconv, 0, sizeof(int), c->stream));
+  return 0;
+}
+
+// Fails if a Rayleigh-Ritz eigensolve since the last clear did not converge.
+int gs_flag_check(tdhf_ctx* c) {
+  int f = 0;
+  HIPCHK(c, hipMemcpyAsync(&f, c->gs.nonconv, sizeof f, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (f)
+    return fail(c, TDHF_ENUMERIC,
+                "subspace eigensolver (cyclic Jacobi) did not converge in 100 sweeps at k-point %d",
+                f - 1);
   return 0;
 }
 
@@ -43,7 +64,7 @@ int launch_rr(tdhf_ctx* c, const zd* W, const zd* HW, size_t blk, int m, zd* uou
     HIPCHK(c, hipFuncSetAttribute((const void*)tdhf::rr_kernel,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(tdhf::rr_kernel, dim3(c->nk), dim3(256), lds, c->stream, W, HW, blk, c->ng,
-                     c->nb, m, c->dv, uout, (double*)nullptr);
+                     c->nb, m, c->dv, uout, (double*)nullptr, c->gs.nonconv);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -84,6 +105,7 @@ extern "C" int tdhf_gs_run(tdhf_ctx* c, double* u, int nscf, int ncg, double tol
   if (m > 64) return fail(c, TDHF_EINVAL, "NB*NCG=%d exceeds 64", m);
   if (m > c->ng) return fail(c, TDHF_EINVAL, "NB*NCG=%d exceeds NG=%d", m, c->ng);
   CHKRET(gs_alloc(c, ncg));
+  CHKRET(gs_flag_clear(c));
   CHKRET(upload(c, c->d_u0, u));
   double r[7];
   CHKRET(observables_of(c, c->d_u0, 0.0, r));
@@ -102,6 +124,7 @@ extern "C" int tdhf_gs_run(tdhf_ctx* c, double* u, int nscf, int ncg, double tol
     eprev = r[0];
     if (conv) break;
   }
+  CHKRET(gs_flag_check(c));
   if (niter) *niter = it > nscf ? nscf : it;
   if (epsbk || epserr) {
     CHKRET(tdhf::ctx_apply(c, c->d_u0, c->d_u0, c->nb, 0.0, c->gs.HW));

@@ -276,30 +276,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   double* sRed = sK2 + NG;
   double* sPart = sRed + 16;  // apply_part_doubles(nw, NG)
 
-  // field values for this launch
-  int step = 0, slot = a.obs_slot;
-  double Akin = a.A, Aobs = a.Aobs;
-  if (a.step_base) {
-    step = *a.step_base + a.jstep;
-    const double At = gload(Aarr + step), Aav = gload(Aavearr + step);
-    if (a.mode == M_S1) {
-      Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
-      Aobs = step > 0 ? gload(Aarr + step - 1) : 0.0;
-      slot = step;
-    } else if (a.mode == M_S2 || a.mode == M_S3) {
-      Akin = Aav;
-    } else if (a.mode == M_S4) {
-      Akin = At;
-    } else {  // M_OBS after step 'step'
-      Akin = At;
-      Aobs = At;
-      slot = step + 1;
-    }
-  }
   // taped forward: the stage states live in the tape slots of this step
   zd* tape0 = nullptr;
   if (a.tape && a.step_base) {
-    zd* tp = a.tape + (size_t)(step % a.tape_seg) * 4 * a.nstate;
+    const int tstep = *a.step_base + a.jstep;
+    zd* tp = a.tape + (size_t)(tstep % a.tape_seg) * 4 * a.nstate;
     switch (a.mode) {
       case M_S1: tape0 = tp; unext = tp + a.nstate; break;
       case M_S2: u = w = tp + a.nstate; unext = tp + 2 * a.nstate; break;
@@ -320,24 +301,16 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     if (obs) obs += (size_t)tr * a.ostride;
   }
 
-  // ---- prologue: w of this row and the kinetic kernels into LDS ----
+  // ---- prologue: every operand load of the first pass is issued before the
+  // first wait, so the loop starts after one memory round trip: the lane
+  // twiddles, the first u(q,j), then w of this row and the kinetic kernels,
+  // which go to LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
-  if (tid < NG) {
-    sW[tid] = gload(w + o);
-    sK2[tid] = gload(kc + NG + tid);
-    sK1[tid] = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
-  }
-  const double kxv = gload(kx + ik);
   F fft;
   fft.init(a.tw, l);
 
   zd acc[R], un[R];
   double den[R];
-  static_for<0, R>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    acc[m] = mk(0.0, 0.0);
-    den[m] = 0.0;
-  });
 
   // ---- exchange pair loop (ubkwbk_vFwbk, src/Functions.f90:247-257) ----
   // u(q,j) of the next pair is fetched while the current pair is transformed;
@@ -355,6 +328,24 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     });
   };
   if (grp < nitem) fetch(cq, cj);
+  zd w_o = mk(0.0, 0.0), k1_o = mk(0.0, 0.0);
+  double k2_o = 0.0;
+  if (tid < NG) {
+    w_o = gload(w + o);
+    k2_o = gload(kc + NG + tid);
+    k1_o = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
+  }
+  const double kxv = gload(kx + ik);
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    acc[m] = mk(0.0, 0.0);
+    den[m] = 0.0;
+  });
+  if (tid < NG) {
+    sW[tid] = w_o;
+    sK2[tid] = k2_o;
+    sK1[tid] = k1_o;
+  }
   // When the whole grid is one workgroup per CU, the two waves sharing a SIMD
   // belong to this workgroup and would otherwise finish far apart (the older
   // one is issued first whenever both are ready, and the other then runs the
@@ -414,6 +405,29 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     cj = nj;
   }
   if (a.balance) set_wave_priority(0);
+
+  // field values of this launch: read only now, so that the device step
+  // counter and the field arrays (a chain of two dependent loads) do not
+  // hold back the operand loads of the pair loop
+  int slot = a.obs_slot;
+  double Akin = a.A, Aobs = a.Aobs;
+  if (a.step_base) {
+    const int step = *a.step_base + a.jstep;
+    const double At = gload(Aarr + step), Aav = gload(Aavearr + step);
+    if (a.mode == M_S1) {
+      Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
+      Aobs = step > 0 ? gload(Aarr + step - 1) : 0.0;
+      slot = step;
+    } else if (a.mode == M_S2 || a.mode == M_S3) {
+      Akin = Aav;
+    } else if (a.mode == M_S4) {
+      Akin = At;
+    } else {  // M_OBS after step 'step'
+      Akin = At;
+      Aobs = At;
+      slot = step + 1;
+    }
+  }
 
   // epilogue operands: loaded now, used after the reduction (whose barriers
   // order LDS only, so these loads stay in flight meanwhile)
@@ -608,29 +622,52 @@ __global__ void twiddle_kernel(zd* tw) {
   if (i < 2 * NG) tw[i] = GroupFFT<NG>::table_value(i);
 }
 
-// Sums the per-row partials of each observables slot into
+// Compensated (Neumaier) sum: the rounding error of every addition is kept in
+// c, so the result carries about one rounding of the total instead of one per
+// term.  The energy objective of the adjoint is a difference of two such
+// totals, and its finite-difference check divides that difference by h.
+struct CompSum {
+  double s = 0.0, c = 0.0;
+  __device__ __forceinline__ void add(double x) {
+    const double t = s + x;
+    c += fabs(s) >= fabs(x) ? (s - t) + x : (x - t) + s;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return s + c; }
+};
+
+// Sums the per-row partials of one observables slot p (rows rows) into
 // (Etot, N, J, Ekin, Eion, EH, Ex), src/Functions.f90:74-91 and :289-318.
+__device__ __forceinline__ void obs_reduce_slot(const double* p, int rows, double dv, double vcell,
+                                                double* o) {
+  CompSum ek, jj, ex;
+  for (int r = 0; r < rows; ++r) {
+    ek.add(p[3 * r + 0]);
+    jj.add(p[3 * r + 1]);
+    ex.add(p[3 * r + 2]);
+  }
+  const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
+  CompSum et;
+  et.add(ek.s);
+  et.add(ei);
+  et.add(eh);
+  et.add(0.5 * ex.s);
+  et.add(ek.c);
+  et.add(0.5 * ex.c);
+  o[0] = et.value();
+  o[1] = n;
+  o[2] = -(jj.value() * dv / vcell);
+  o[3] = ek.value();
+  o[4] = ei;
+  o[5] = eh;
+  o[6] = 0.5 * ex.value();
+}
+
 __global__ void obs_reduce_kernel(const double* obs, int stride, int rows, int nslots, double dv,
                                   double vcell, double* out) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
-  const double* p = obs + (size_t)s * stride;
-  double ek = 0.0, jj = 0.0, ex = 0.0;
-  for (int r = 0; r < rows; ++r) {
-    ek += p[3 * r + 0];
-    jj += p[3 * r + 1];
-    ex += p[3 * r + 2];
-  }
-  ex = 0.5 * ex;
-  const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
-  double* o = out + (size_t)7 * s;
-  o[0] = ek + ei + eh + ex;
-  o[1] = n;
-  o[2] = -(jj * dv / vcell);
-  o[3] = ek;
-  o[4] = ei;
-  o[5] = eh;
-  o[6] = ex;
+  obs_reduce_slot(obs + (size_t)s * stride, rows, dv, vcell, out + (size_t)7 * s);
 }
 
 // Batched variant: one thread per (trajectory, slot); out[(t*nslots + s)*7].
@@ -639,24 +676,8 @@ __global__ void obs_reduce_batch_kernel(const TrajDesc* td, int ntraj, int nb, i
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntraj * nslots) return;
   const int t = i / nslots, s = i - t * nslots;
-  const int rows = nb * td[t].nk;
-  const double* p = td[t].obs + (size_t)s * td[t].obs_stride;
-  double ek = 0.0, jj = 0.0, ex = 0.0;
-  for (int r = 0; r < rows; ++r) {
-    ek += p[3 * r + 0];
-    jj += p[3 * r + 1];
-    ex += p[3 * r + 2];
-  }
-  ex = 0.5 * ex;
-  const double n = p[3 * rows + 0], ei = p[3 * rows + 1], eh = p[3 * rows + 2];
-  double* o = out + (size_t)7 * i;
-  o[0] = ek + ei + eh + ex;
-  o[1] = n;
-  o[2] = -(jj * dv / vcell);
-  o[3] = ek;
-  o[4] = ei;
-  o[5] = eh;
-  o[6] = ex;
+  obs_reduce_slot(td[t].obs + (size_t)s * td[t].obs_stride, nb * td[t].nk, dv, vcell,
+                  out + (size_t)7 * i);
 }
 
 __global__ void step_advance_kernel(int* step_base, int by) {

@@ -92,7 +92,7 @@ extern "C" int tdhf_gs_run_diag(tdhf_ctx* c, double* u, int nscf, int ncg, doubl
     HIPCHK(c, hipGetLastError());
     return 0;
   };
-  if ((rc = upload(c, c->d_u0, u))) return done(rc);
+  if ((rc = gs_flag_clear(c)) || (rc = upload(c, c->d_u0, u))) return done(rc);
   double r[7];
   if ((rc = observables_of(c, c->d_u0, 0.0, r))) return done(rc);
   if (etot_hist) etot_hist[0] = r[0];
@@ -114,7 +114,7 @@ extern "C" int tdhf_gs_run_diag(tdhf_ctx* c, double* u, int nscf, int ncg, doubl
     e = hipMemcpyAsync(diag, ddiag, (size_t)5 * nscf * sizeof(double), hipMemcpyDeviceToHost,
                        c->stream);
   if (e != hipSuccess) return done(fail(c, TDHF_EDEVICE, "copy: %s", hipGetErrorString(e)));
-  if ((rc = sync(c))) return done(rc);
+  if ((rc = sync(c)) || (rc = gs_flag_check(c))) return done(rc);
   if (epsbk) std::memcpy(epsbk, h.data(), rows * sizeof(double));
   if (epserr) std::memcpy(epserr, h.data() + rows, rows * sizeof(double));
   return done(download(c, u, c->d_u0));

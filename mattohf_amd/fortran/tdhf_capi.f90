@@ -85,6 +85,38 @@ module tdhf_capi
       real(c_double), intent(out) :: etot(*), nele(*), jtot(*), init3(3)
     end function
 
+    ! the RT loop in pieces on the device-resident state and field
+    integer(c_int) function tdhf_state_upload(ctx, u) bind(C, name="tdhf_state_upload")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(in) :: u(*)
+    end function
+
+    integer(c_int) function tdhf_state_download(ctx, u) bind(C, name="tdhf_state_download")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: ctx
+      complex(c_double_complex), intent(out) :: u(*)
+    end function
+
+    integer(c_int) function tdhf_rt_begin(ctx, nt, A, Aave) bind(C, name="tdhf_rt_begin")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(*), Aave(*)
+    end function
+
+    integer(c_int) function tdhf_rt_advance(ctx, n) bind(C, name="tdhf_rt_advance")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n
+    end function
+
+    integer(c_int) function tdhf_rt_end(ctx, etot, nele, jtot, init3) bind(C, name="tdhf_rt_end")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: etot(*), nele(*), jtot(*), init3(3)
+    end function
+
     integer(c_int) function tdhf_gs_run(ctx, u, nscf, ncg, tol, etot_hist, epsbk, epserr, &
          niter) bind(C, name="tdhf_gs_run")
       import :: c_int, c_double, c_double_complex, c_ptr

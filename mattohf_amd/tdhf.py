@@ -54,6 +54,9 @@ SIGNATURES = {
     "tdhf_state_upload": (_i, [_vp, _vp]),
     "tdhf_state_download": (_i, [_vp, _vp]),
     "tdhf_rt_propagate": (_i, [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "tdhf_rt_begin": (_i, [_vp, _i, _vp, _vp]),
+    "tdhf_rt_advance": (_i, [_vp, _i]),
+    "tdhf_rt_end": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "tdhf_gs_run": (_i, [_vp, _vp, _i, _i, _d, _vp, _vp, _vp, C.POINTER(_i)]),
     "tdhf_gram_schmidt": (_i, [_vp, _vp]),
     "tdhf_set_kernel_timing": (_i, [_vp, _i]),
@@ -314,6 +317,25 @@ class TDHF:
         _chk(load().tdhf_rt_propagate(self.h, nt, _ptr(A), _ptr(Aave), _ptr(e), _ptr(n),
                                       _ptr(j), _ptr(i3)), self.h)
         return e, n, j, i3
+
+    def rt_begin(self, A, Aave):
+        """Uploads the field of a run (tdhf_rt_begin); the resident state is the
+        one last uploaded or propagated."""
+        A, Aave = _f64(A), _f64(Aave, np.size(A))
+        _chk(load().tdhf_rt_begin(self.h, A.size, _ptr(A), _ptr(Aave)), self.h)
+        self._rt_nt, self._rt_done = A.size, 0
+
+    def rt_advance(self, n):
+        """The next n steps of the run opened by rt_begin (tdhf_rt_advance)."""
+        _chk(load().tdhf_rt_advance(self.h, int(n)), self.h)
+        self._rt_done += int(n)
+
+    def rt_end(self):
+        """Observables of the steps advanced since rt_begin: (etot, nele, jtot, init3)."""
+        n = getattr(self, "_rt_done", 0)
+        e, nn, j, i3 = np.empty(n), np.empty(n), np.empty(n), np.empty(3)
+        _chk(load().tdhf_rt_end(self.h, _ptr(e), _ptr(nn), _ptr(j), _ptr(i3)), self.h)
+        return e, nn, j, i3
 
     def GS_calculation(self, ubk, nscf, ncg, tol=0.0):
         """Returns (ubk, Etot history[0..niter], epsbk(NB,NK), epserrorbk(NB,NK))."""

@@ -1,8 +1,9 @@
 """Pins the CPU oracle (oracle/tdhf_oracle.c) -- CPU only, no GPU.
 
 The reference ships no tests, fixtures or golden vectors (its tree has no test
-files), and its Fortran build needs FFTW3's fftw3.f include, MPI and LAPACK,
-which this image does not provide, so it is not run here.  Each oracle
+files), and its Fortran build needs FFTW3's fftw3.f include file, which this
+image does not have (MPI and MKL's FFTW3/LAPACK symbols are present; a
+stand-in header would not make an acceptable oracle), so it is not run here.  Each oracle
 function is therefore checked against something independent of its code:
 
   orc_fft            numpy.fft (FFTW sign / normalisation) incl. a non-2^n size
@@ -22,7 +23,15 @@ function is therefore checked against something independent of its code:
   orc_ecomps         HF identity sum(occ*eps) = Ekin + Eion + 2 EH + 2 Ex;
                      Ekin by Parseval
   orc_je             diamagnetic identity J(A) = J(0) - A N / a; J(GS) = 0
-  orc_rk4_step       4th-order convergence (A=0 energy drift ratio ~ 2^4)
+  orc_rk4_step       4th-order convergence (A=0 energy drift ratio ~ 2^4);
+                     the stage fields (2 Aave - A, Aave, Aave, A): with
+                     plane-wave orbitals and no ions every stage state stays a
+                     plane wave per (k, band), so one step is a scalar
+                     recursion over the amplitudes with the closed-form
+                     kinetic and exchange eigenvalues
+  orc_epsbk          band energy Re<u|H u> dv and residual sqrt(sum |r| dv)
+                     (sum of moduli, not of squares) from H|u> on a
+                     non-eigenstate
   orc_heev           numpy.linalg.eigh
   orc_ks_krylov      Ritz subspace invariant under rotation of the input bands
   orc_gs_run         converged residuals; eigenvalues independent of NCG
@@ -365,6 +374,86 @@ def test_rk4_fourth_order(gs_cfg1):
         _, e, _, _, _ = O.RT_calculation(up, dt, z, z)
         drift.append(abs(e[-1] - e0))
     assert drift[0] > 0 and 8.0 < drift[0] / drift[1] < 40.0
+
+
+def test_rk4_stage_fields_plane_waves():
+    """RT_RK4_ubk (src/RT_calculation.f90:154-173) evaluates its four stages at
+    A(it-1) = 2 Aave - A, Aave, Aave and A.  Orbitals u_kb = c_kb e^{i G_kb x}/sqrt(a)
+    with vion = 0 have a uniform density (v_H = 0 since vCG(0) = 0), and the
+    exchange maps e^{i G_kb x} onto itself with the eigenvalue
+    -(1/(NK a)) sum_qj |c_qj|^2 v(G_kb - G_qj + k - q), so every stage state is a
+    plane wave per (k, b) and the step is a recursion over the amplitudes c_kb.
+    Any other assignment of the stage fields moves the result by O(dt dA)."""
+    ng, nk, nb, a = 16, 3, 2, 5.0
+    S = po.System(ng, nk, nb, nb, a)
+    O = po.Oracle(S, np.zeros(ng))
+    _, gx, kx = S.grids()
+    vk = S.vCGkk()
+    gi = np.array([[1, 14], [8, 0], [15, 6]])  # FFT index of each (k, band); 8 = Nyquist
+    rng = np.random.default_rng(21)
+    c0 = (0.8 + 0.4 * rng.random((nk, nb))) * np.exp(2j * np.pi * rng.random((nk, nb)))
+    x = np.arange(ng)
+    u = np.zeros(S.shape, dtype=complex, order="F")
+    for ik in range(nk):
+        for ib in range(nb):
+            u[:, ib, ik] = c0[ik, ib] * np.exp(2j * np.pi * gi[ik, ib] * x / ng) / math.sqrt(a)
+
+    def h(c, A):
+        out = np.empty_like(c)
+        for ik in range(nk):
+            for ib in range(nb):
+                lam = 0.0
+                for iq in range(nk):
+                    for jb in range(nb):
+                        lam += abs(c[iq, jb]) ** 2 * vk[ik * nk + iq][(gi[ik, ib] - gi[iq, jb]) % ng]
+                out[ik, ib] = 0.5 * (gx[gi[ik, ib]] + kx[ik] + A) ** 2 - lam / (nk * a)
+        return out
+
+    def step(c, dt, A1, A2, A3, A4):
+        k1 = -1j * h(c, A1) * c
+        y = c + 0.5 * dt * k1
+        k2 = -1j * h(y, A2) * y
+        y = c + 0.5 * dt * k2
+        k3 = -1j * h(y, A3) * y
+        y = c + dt * k3
+        k4 = -1j * h(y, A4) * y
+        return c + dt / 6.0 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+    dt, aave, A = 0.05, 0.1, 0.3
+    got = O.RT_RK4_ubk(u, dt, aave, A)
+    cg = np.array([[np.vdot(u[:, ib, ik] / c0[ik, ib], got[:, ib, ik]) * a / ng
+                    for ib in range(nb)] for ik in range(nk)])
+    want = step(c0, dt, 2 * aave - A, aave, aave, A)
+    np.testing.assert_allclose(cg, want, rtol=0, atol=1e-13)
+    # the state stays a plane wave per (k, b)
+    for ik in range(nk):
+        for ib in range(nb):
+            pw = cg[ik, ib] * np.exp(2j * np.pi * gi[ik, ib] * x / ng) / math.sqrt(a)
+            np.testing.assert_allclose(got[:, ib, ik], pw, rtol=0, atol=1e-13)
+    # the recursion separates the conventions: each alternative is far off
+    for alt in ((A, aave, aave, A), (2 * aave - A, A, A, A), (aave, aave, aave, aave),
+                (2 * aave - A, aave, aave, aave)):
+        assert np.abs(step(c0, dt, *alt) - want).max() > 1e-5
+
+
+def test_epsbk_residual_is_sum_of_moduli():
+    """ubk_epserrorbk (src/Functions.f90:340-358) takes sqrt(sum |r| dv) of the
+    residual r = H u - eps u, with eps = Re<u|H u> dv (ubk_epsbk, :321-337):
+    the sum of moduli, not of squared moduli.  On a non-eigenstate the two
+    differ by orders of magnitude."""
+    S, O, _ = make(32, 3, 3, 2)
+    u = rand_orb(S, 22)
+    eps, err = O.ubk_epsbk(u)
+    hu = O.ubkwbk_hwbk(u, u, 0.0)
+    dv = 8.0 / 32
+    for ik in range(3):
+        for ib in range(3):
+            e = np.vdot(u[:, ib, ik], hu[:, ib, ik]).real * dv
+            r = hu[:, ib, ik] - e * u[:, ib, ik]
+            assert abs(eps[ib, ik] - e) < 1e-12 * max(1.0, abs(e))
+            assert abs(err[ib, ik] - math.sqrt(np.abs(r).sum() * dv)) < 1e-12 * err[ib, ik]
+            sq = math.sqrt((np.abs(r) ** 2).sum() * dv)
+            assert abs(err[ib, ik] - sq) > 0.1 * err[ib, ik]
 
 
 def test_afield_lp_closed_form():
