@@ -96,6 +96,20 @@ int tdhf_observables(tdhf_ctx* ctx, const double* u, double A, double* etot, dou
  * (src/RT_calculation.f90:154-173): stages at A(it-1)=2*Aave-A, Aave, Aave, A. */
 int tdhf_rk4_step(tdhf_ctx* ctx, double* u, double Aave, double A);
 
+/* tdhf_rk4_step for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  u is a DEVICE pointer to the state
+ * (NG*NB*NK complex, reference layout), advanced in place by one RK4 step at
+ * A(it-1) = 2*Aave - A, Aave, Aave, A; the four stage launches are enqueued on
+ * stream (a hipStream_t; NULL = the context's stream) and the call returns
+ * without waiting.  The stage buffers live in a fixed device array of the
+ * library (6 MiB per GPU, held until the process ends): NG*NB*NK must not
+ * exceed 131072 (TDHF_EINVAL otherwise).  The call allocates nothing; the
+ * context (tables) must have been created before.  All calls on one GPU share
+ * that array, so calls whose launches overlap on one GPU (other streams,
+ * other host threads) race. */
+int tdhf_rk4_step_noalloc(tdhf_ctx* ctx, double* u, double Aave, double A, void* stream);
+
 /* nt RK4 steps with the observables after every step.  Replaces the loop of
  * RT_calculation (src/RT_calculation.f90:29-60).  u is propagated in place.
  *   A, Aave          : nt values (Define_Afield output, src/RT_calculation.f90

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +31,19 @@ using tdhf::zd;
 namespace {
 
 thread_local std::string g_err = "no error";
+
+// Stage buffers of the allocation-free RK4 step (tdhf_rk4_step_noalloc):
+// ksum | sa | sb, each of at most kNoAllocMaxState complex elements
+// (NG*NB*NK <= 131072, e.g. NG=128, NB=8, NK=128), 6 MiB in all.  A
+// file-scope device array: every GPU the library runs on holds its own copy
+// until the process ends.  Host code never names the array in a kernel
+// argument or a copy; it uses the device address taken with
+// hipGetSymbolAddress for each device (noalloc_scratch).
+constexpr size_t kNoAllocMaxState = (size_t)1 << 17;
+__device__ __align__(256) double scratch[6 * kNoAllocMaxState];
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;  // per device id
 
 }  // namespace
 
@@ -125,6 +139,26 @@ int check_device(tdhf_ctx* c) {
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(c, TDHF_EDEVICE, "libtdhf is built for gfx950, device is %s", prop.gcnArchName);
   return 0;
+}
+
+// Device address of the allocation-free step's scratch array on the current
+// device (which is c->device), taken once per device.
+int noalloc_scratch_init(tdhf_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= c->device) g_scratch_ptr.resize(c->device + 1, nullptr);
+  if (g_scratch_ptr[c->device]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(c, TDHF_EDEVICE, "hipGetSymbolAddress of the step scratch failed: %s",
+                hipGetErrorString(e));
+  g_scratch_ptr[c->device] = static_cast<double*>(ptr);
+  return 0;
+}
+
+double* noalloc_scratch(const tdhf_ctx* c) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return c->device < (int)g_scratch_ptr.size() ? g_scratch_ptr[c->device] : nullptr;
 }
 
 template <typename T>
@@ -688,6 +722,7 @@ int tdhf_create(const tdhf_params* p, const double* vion, const double* vcg, con
       return bail(TDHF_EDEVICE);
     }
   }
+  if ((rc = noalloc_scratch_init(c))) return bail(rc);
   *out = c;
   return 0;
 }
@@ -883,6 +918,48 @@ int tdhf_rk4_step(tdhf_ctx* c, double* u, double Aave, double A) {
   CHKRET(enqueue_steps(c, 0, 1));
   CHKRET(sync(c));
   return tdhf_state_download(c, u);
+}
+
+int tdhf_rk4_step_noalloc(tdhf_ctx* c, double* u, double Aave, double A, void* stream) {
+  if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
+  if (!u) return fail(c, TDHF_EINVAL, "NULL array");
+  const size_t n = nstate(c);
+  if (n > kNoAllocMaxState)
+    return fail(c, TDHF_EINVAL,
+                "state of %zu complex elements (NG*NB*NK) exceeds the allocation-free step's "
+                "limit of %zu",
+                n, kNoAllocMaxState);
+  double* sc = noalloc_scratch(c);
+  if (!sc) return fail(c, TDHF_EDEVICE, "step scratch not set up on device %d", c->device);
+  zd* ksum = reinterpret_cast<zd*>(sc);
+  zd* sa = ksum + kNoAllocMaxState;
+  zd* sb = sa + kNoAllocMaxState;
+  zd* u0 = reinterpret_cast<zd*>(u);
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  // the four stages of RT_RK4_ubk with the fields as kernel arguments
+  // (no step counter, no field arrays, no observables)
+  tdhf::ApplyArgs a = base_args(c);
+  a.u0 = u0;
+  a.ksum = ksum;
+  const int rows = c->nb * c->nk;
+  const int modes[4] = {tdhf::M_S1, tdhf::M_S2, tdhf::M_S3, tdhf::M_S4};
+  const double fields[4] = {2.0 * Aave - A, Aave, Aave, A};
+  zd* ins[4] = {u0, sa, sb, sa};
+  zd* nexts[4] = {sa, sb, sa, nullptr};
+  for (int s = 0; s < 4; ++s) {
+    a.mode = modes[s];
+    a.A = a.Aobs = fields[s];
+    a.u = a.w = ins[s];
+    a.unext = nexts[s];
+    switch (c->ng) {
+      case 16: launch_apply_t<16>(a, rows, 1, c->nw, c->lds, st); break;
+      case 32: launch_apply_t<32>(a, rows, 1, c->nw, c->lds, st); break;
+      case 64: launch_apply_t<64>(a, rows, 1, c->nw, c->lds, st); break;
+      case 128: launch_apply_t<128>(a, rows, 1, c->nw, c->lds, st); break;
+    }
+    HIPCHK(c, hipGetLastError());
+  }
+  return 0;
 }
 
 int tdhf_gram_schmidt(tdhf_ctx* c, double* u) {

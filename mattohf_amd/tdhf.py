@@ -50,6 +50,7 @@ SIGNATURES = {
     "tdhf_hpsi": (_i, [_vp, _vp, _vp, _d, _vp]),
     "tdhf_observables": (_i, [_vp, _vp, _d, _vp, _vp, _vp, _vp]),
     "tdhf_rk4_step": (_i, [_vp, _vp, _d, _d]),
+    "tdhf_rk4_step_noalloc": (_i, [_vp, _vp, _d, _d, _vp]),
     "tdhf_rt_run": (_i, [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "tdhf_state_upload": (_i, [_vp, _vp]),
     "tdhf_state_download": (_i, [_vp, _vp]),
