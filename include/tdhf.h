@@ -291,11 +291,20 @@ int tdhf_batch_run(const tdhf_params* p, int ntraj, const int* nk, const double*
  * least NG/64; default the smallest).
  * tdhf_batch_autotune times nt_trial steps for each candidate on a copy of the
  * state (the batch state is unchanged), keeps the fastest and remembers it for
- * batches of the same shape created later in the process. */
+ * batches of the same shape created later in the process; best_nw receives
+ * its waves per workgroup (tdhf_batch_get_schedule its row schedule). */
 int tdhf_batch_set_waves(tdhf_batch* b, int nw);
 int tdhf_batch_get_waves(const tdhf_batch* b);
 int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
                         int* best_nw);
+/* Row schedule of the batched launch (which workgroup runs which trajectory's
+ * row, and so on which XCD): 0 = trajectories bin-packed over the XCDs (a
+ * trajectory's rows on one XCD, contiguous; the default), 1 = the same with
+ * the rows of an XCD's trajectories interleaved, 2 = no XCD pinning.  The
+ * results do not depend on it.  tdhf_batch_autotune times every (schedule,
+ * waves per workgroup) pair and keeps the fastest pair. */
+int tdhf_batch_set_schedule(tdhf_batch* b, int sched);
+int tdhf_batch_get_schedule(const tdhf_batch* b);
 int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);
 int tdhf_batch_kernel_stats(tdhf_batch* b, long* launches, double* total_ms);
 

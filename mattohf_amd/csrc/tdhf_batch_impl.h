@@ -43,7 +43,9 @@ struct tdhf_batch {
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
   int graph_nw = 0;
+  int graph_sched = 0;
   int nw = 4;
+  int sched = 0;  // row schedule (build_schedule)
   bool timing = false;
   long launches = 0;
   double total_ms = 0.0;
@@ -90,15 +92,31 @@ int batch_valid_nw(const tdhf_batch* b, int nw) {
   return nw >= 1 && nw <= 8 && 64 * nw >= b->p.ng;
 }
 
-// Schedule: LPT over XCDs by trajectory cost, rows longest-first per XCD,
-// interleaved so that entry b goes to XCD b % 8.
-void build_schedule(const tdhf_batch* b, std::vector<int2>& map) {
+constexpr int kNumSchedules = 3;
+
+// Row schedules (workgroup b of a launch is dispatched to XCD b % 8):
+//  0: trajectories bin-packed over the 8 XCDs by cost (longest-processing-
+//     time first), each XCD's queue longest trajectory first with its rows
+//     contiguous, queues interleaved so that entry b is on XCD b % 8: a
+//     trajectory's orbitals stay in one XCD's L2;
+//  1: as 0, but each XCD's queue takes its trajectories' rows round-robin
+//     (row r of every trajectory before row r+1), so the rows running at once
+//     on an XCD belong to different trajectories;
+//  2: no XCD pinning: all trajectories longest first, rows contiguous, in
+//     dispatch order (a trajectory's rows spread over the XCDs).
+void build_schedule(const tdhf_batch* b, std::vector<int2>& map, int sched) {
   constexpr int kXcd = 8;
   const int nb = b->p.nb;
   std::vector<int> order(b->ntraj);
   std::iota(order.begin(), order.end(), 0);
   auto cost = [&](int t) { return (double)b->nk[t] * b->nk[t] * nb * b->p.nbocc + b->nk[t] * nb; };
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  map.clear();
+  if (sched == 2) {
+    for (int t : order)
+      for (int r = 0; r < nb * b->nk[t]; ++r) map.push_back(make_int2(t, r));
+    return;
+  }
   std::vector<double> load(kXcd, 0.0);
   std::vector<std::vector<int>> bins(kXcd);
   for (int t : order) {
@@ -109,14 +127,39 @@ void build_schedule(const tdhf_batch* b, std::vector<int2>& map) {
   std::vector<std::vector<int2>> q(kXcd);
   size_t longest = 0;
   for (int x = 0; x < kXcd; ++x) {
-    for (int t : bins[x])
-      for (int r = 0; r < nb * b->nk[t]; ++r) q[x].push_back(make_int2(t, r));
+    if (sched == 1) {
+      int rmax = 0;
+      for (int t : bins[x]) rmax = std::max(rmax, nb * b->nk[t]);
+      for (int r = 0; r < rmax; ++r)
+        for (int t : bins[x])
+          if (r < nb * b->nk[t]) q[x].push_back(make_int2(t, r));
+    } else {
+      for (int t : bins[x])
+        for (int r = 0; r < nb * b->nk[t]; ++r) q[x].push_back(make_int2(t, r));
+    }
     longest = std::max(longest, q[x].size());
   }
-  map.clear();
   for (size_t i = 0; i < longest; ++i)
     for (int x = 0; x < kXcd; ++x) map.push_back(i < q[x].size() ? q[x][i] : make_int2(-1, 0));
   while (!map.empty() && map.back().x < 0) map.pop_back();
+}
+
+// Installs row schedule sched (device row map; the step graph is recaptured).
+int batch_set_schedule(tdhf_batch* b, int sched) {
+  std::vector<int2> map;
+  build_schedule(b, map, sched);
+  if ((int)map.size() > b->ngrid) {
+    dfree(b->d_rowmap);
+    CHKRET(balloc(b, &b->d_rowmap, map.size()));
+  }
+  BHIP(b, hipMemcpyAsync(b->d_rowmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice,
+                         b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->ngrid = (int)map.size();
+  b->sched = sched;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  return 0;
 }
 
 int batch_update_desc(tdhf_batch* b) {
@@ -212,7 +255,7 @@ int batch_enqueue_steps(tdhf_batch* b, int n, int nw) {
 int batch_run_steps(tdhf_batch* b, int nt) {
   int done = 0;
   if (!b->timing && nt >= kGraphSteps) {
-    if (b->gexec && b->graph_nw != b->nw) {
+    if (b->gexec && (b->graph_nw != b->nw || b->graph_sched != b->sched)) {
       (void)hipGraphExecDestroy(b->gexec);
       b->gexec = nullptr;
     }
@@ -234,6 +277,7 @@ int batch_run_steps(tdhf_batch* b, int nt) {
       if (e != hipSuccess) return bfail(b, TDHF_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
       b->graph_steps = kGraphSteps;
       b->graph_nw = b->nw;
+      b->graph_sched = b->sched;
     }
     while (nt - done >= b->graph_steps) {
       BHIP(b, hipGraphLaunch(b->gexec, b->stream));
@@ -285,9 +329,9 @@ void batch_release(tdhf_batch* b) {
   b->stream = nullptr;
 }
 
-// autotuner cache: shape -> waves per workgroup
-std::map<std::vector<int>, int>& tune_cache() {
-  static std::map<std::vector<int>, int> m;
+// autotuner cache: shape -> (waves per workgroup, row schedule)
+std::map<std::vector<int>, std::pair<int, int>>& tune_cache() {
+  static std::map<std::vector<int>, std::pair<int, int>> m;
   return m;
 }
 
@@ -365,22 +409,17 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
     bfail(b, TDHF_EDEVICE, "vion upload failed");
     return bail(TDHF_EDEVICE);
   }
-  std::vector<int2> map;
-  build_schedule(b, map);
-  b->ngrid = (int)map.size();
-  if ((rc = balloc(b, &b->d_rowmap, map.size()))) return bail(rc);
-  if (hipMemcpy(b->d_rowmap, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice) !=
-      hipSuccess) {
-    bfail(b, TDHF_EDEVICE, "schedule upload failed");
-    return bail(TDHF_EDEVICE);
-  }
+  // launch shape and row schedule: the cached autotuner choice, else the
+  // fewest waves per workgroup that cover the grid (one at NG <= 64: the
+  // groups of a single wave need no workgroup barrier, and eight such
+  // workgroups share a CU) and the XCD-pinned schedule
+  auto ct = tune_cache().find(tune_key(b));
+  b->nw = ct != tune_cache().end() ? ct->second.first : (p->ng + 63) / 64;
+  b->ngrid = 0;
+  if ((rc = batch_set_schedule(b, ct != tune_cache().end() ? ct->second.second : 0)))
+    return bail(rc);
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
-  // launch shape: cached autotuner choice, else the fewest waves per
-  // workgroup that cover the grid (one at NG <= 64: the groups of a single wave
-  // need no workgroup barrier, and eight such workgroups share a CU)
-  auto ct = tune_cache().find(tune_key(b));
-  b->nw = ct != tune_cache().end() ? ct->second : (p->ng + 63) / 64;
   {
     const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
                      : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
@@ -491,6 +530,14 @@ int tdhf_batch_set_waves(tdhf_batch* b, int nw) {
 
 int tdhf_batch_get_waves(const tdhf_batch* b) { return b ? b->nw : 0; }
 
+int tdhf_batch_set_schedule(tdhf_batch* b, int sched) {
+  if (!b || sched < 0 || sched >= kNumSchedules)
+    return bfail(b, TDHF_EINVAL, "invalid row schedule %d", sched);
+  return batch_set_schedule(b, sched);
+}
+
+int tdhf_batch_get_schedule(const tdhf_batch* b) { return b ? b->sched : -1; }
+
 int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
                         int* best_nw) {
   if (!b || nt_trial < 1 || !A || !Aave) return bfail(b, TDHF_EINVAL, "invalid autotune arguments");
@@ -502,32 +549,40 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
     ~FreeOnReturn() { dfree(p); }
   } save_guard{save};
   BHIP(b, hipMemcpyAsync(save, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
-  int best = b->nw;
+  int best = b->nw, best_s = b->sched;
   double best_t = 1e300;
   const int cand[4] = {1, 2, 4, 8};
-  for (int nw : cand) {
-    if (!batch_valid_nw(b, nw)) continue;
-    b->nw = nw;
-    int rc = 0;
-    double dt_ms = 0.0;
-    for (int rep = 0; rep < 2 && !rc; ++rep) {  // first repetition warms up (graph capture)
-      BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
-      BHIP(b, hipStreamSynchronize(b->stream));
-      auto t0 = std::chrono::steady_clock::now();
-      rc = tdhf_batch_propagate(b, nt_trial, A, Aave, nullptr, nullptr, nullptr, nullptr);
-      auto t1 = std::chrono::steady_clock::now();
-      dt_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    }
-    if (rc) return rc;
-    if (dt_ms < best_t) {
-      best_t = dt_ms;
-      best = nw;
+  for (int sc = 0; sc < kNumSchedules; ++sc) {
+    CHKRET(batch_set_schedule(b, sc));
+    for (int nw : cand) {
+      if (!batch_valid_nw(b, nw)) continue;
+      b->nw = nw;
+      int rc = 0;
+      double dt_ms = 1e300;
+      // the first repetition warms up (graph capture); the faster of the
+      // next two counts
+      for (int rep = 0; rep < 3 && !rc; ++rep) {
+        BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice,
+                               b->stream));
+        BHIP(b, hipStreamSynchronize(b->stream));
+        auto t0 = std::chrono::steady_clock::now();
+        rc = tdhf_batch_propagate(b, nt_trial, A, Aave, nullptr, nullptr, nullptr, nullptr);
+        auto t1 = std::chrono::steady_clock::now();
+        if (rep > 0) dt_ms = std::min(dt_ms, std::chrono::duration<double, std::milli>(t1 - t0).count());
+      }
+      if (rc) return rc;
+      if (dt_ms < best_t) {
+        best_t = dt_ms;
+        best = nw;
+        best_s = sc;
+      }
     }
   }
   BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   b->nw = best;
-  tune_cache()[tune_key(b)] = best;
+  CHKRET(batch_set_schedule(b, best_s));
+  tune_cache()[tune_key(b)] = std::make_pair(best, best_s);
   if (best_nw) *best_nw = best;
   return 0;
 }

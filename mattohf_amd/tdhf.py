@@ -82,6 +82,8 @@ SIGNATURES = {
     "tdhf_batch_set_waves": (_i, [_vp, _i]),
     "tdhf_batch_get_waves": (_i, [_vp]),
     "tdhf_batch_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(_i)]),
+    "tdhf_batch_set_schedule": (_i, [_vp, _i]),
+    "tdhf_batch_get_schedule": (_i, [_vp]),
     "tdhf_batch_set_kernel_timing": (_i, [_vp, _i]),
     "tdhf_batch_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
 }
@@ -531,6 +533,12 @@ class Batch:
 
     def waves(self):
         return load().tdhf_batch_get_waves(self.h)
+
+    def set_schedule(self, sched):
+        _bchk(load().tdhf_batch_set_schedule(self.h, int(sched)), self.h)
+
+    def schedule(self):
+        return load().tdhf_batch_get_schedule(self.h)
 
     def autotune(self, A, Aave):
         A = np.ascontiguousarray(A, dtype=np.float64)

@@ -77,6 +77,7 @@ def test_autotune_keeps_state(batch_case):
         B.upload(us)
         best = B.autotune(A[:, :30], Aave[:, :30])
         assert best in (1, 2, 4, 8) and B.waves() == best
+        assert B.schedule() in (0, 1, 2)
         after = B.download()
         for a, b in zip(after, us):
             assert np.array_equal(a, b)

@@ -175,6 +175,22 @@ def test_cfg3_batch_equals_single_runs(cfg3, nw):
         np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("sched", [1, 2])
+def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
+    systems, us, vions, A, Aave, singles = cfg3
+    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+        B.set_schedule(sched)
+        assert B.schedule() == sched
+        B.upload(us)
+        e, n, j, i3 = B.propagate(A, Aave)
+        finals = B.download()
+    for t in range(len(C3["nks"])):
+        u1, e1, n1, j1, i31 = singles[t]
+        np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * np.abs(e1).max())
+        np.testing.assert_allclose(j[t], j1, rtol=0, atol=1e-12 * max(1.0, np.abs(j1).max()))
+        np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
+
+
 def test_cfg3_batch_matches_oracle(cfg3):
     """One trajectory of each k-mesh of the batch against the oracle, over an
     8-step prefix."""
