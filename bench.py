@@ -194,6 +194,7 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
     B.set_kernel_timing(False)
     waves = B.waves()
     sched = B.schedule()
+    xmode = B.exchange()
     B.close()
     f_launch = sum(flops_per_step(ng, k, nb, cfg["nbocc"]) for k in nks) / 4.0
     avg_s = ms / launches / 1e3
@@ -205,6 +206,7 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
         "ms_per_batch_step": wall / steps * 1e3,
         "waves_per_workgroup": waves,
         "row_schedule": sched,
+        "exchange": "pair tiles" if xmode == 1 else "row-wise",
         "avg_launch_us": avg_s * 1e6,
         "achieved_tflops": f_launch / avg_s / 1e12,
         "frac": f_launch / avg_s / 1e12 / PEAK_FP64_TFLOPS,

@@ -305,6 +305,16 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
  * waves per workgroup) pair and keeps the fastest pair. */
 int tdhf_batch_set_schedule(tdhf_batch* b, int sched);
 int tdhf_batch_get_schedule(const tdhf_batch* b);
+/* Exchange evaluation of the batched stages: 0 = row-wise (every ordered pair
+ * of rows (k,i), (q,j) transformed by the fused kernel of row (k,i), as
+ * ubkwbk_vFwbk, src/Functions.f90:247-257), 1 = pair tiles: since the stage
+ * acts on its own state, the convolutions of (k,i),(q,j) and (q,j),(k,i) are
+ * conjugates up to the Nyquist component, so one transform pair per
+ * unordered pair feeds both rows (needs NG = 64 and NB == NBocc; TDHF_EINVAL
+ * otherwise).  Results agree to rounding.  tdhf_batch_autotune includes
+ * both. */
+int tdhf_batch_set_exchange(tdhf_batch* b, int mode);
+int tdhf_batch_get_exchange(const tdhf_batch* b);
 int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);
 int tdhf_batch_kernel_stats(tdhf_batch* b, long* launches, double* total_ms);
 

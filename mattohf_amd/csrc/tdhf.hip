@@ -22,6 +22,7 @@
 
 #include "../../include/tdhf.h"
 #include "tdhf_kernels.h"
+#include "tdhf_pairtile.h"
 #include "tdhf_gs.h"
 #include "tdhf_adjoint.h"
 #include "tdhf_post.h"

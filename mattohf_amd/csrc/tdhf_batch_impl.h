@@ -12,6 +12,7 @@
 #pragma once
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <map>
 #include <numeric>
@@ -46,6 +47,15 @@ struct tdhf_batch {
   int graph_sched = 0;
   int nw = 4;
   int sched = 0;  // row schedule (build_schedule)
+  // exchange: 0 = row-wise in the fused kernel, 1 = pair tiles
+  // (pairtile_kernel, NG = 64 and NB == NBocc) + partial sums in the fused kernel
+  int xmode = 0;
+  int graph_xmode = 0;
+  zd* d_xpart = nullptr;
+  double* d_dpart = nullptr;
+  int4* d_tiles = nullptr;
+  int ntiles = 0;
+  std::vector<size_t> xoff, doff;  // per trajectory (complex / double elements)
   bool timing = false;
   long launches = 0;
   double total_ms = 0.0;
@@ -162,6 +172,80 @@ int batch_set_schedule(tdhf_batch* b, int sched) {
   return 0;
 }
 
+bool batch_pairtile_ok(const tdhf_batch* b) { return b->p.ng == 64 && b->p.nb == b->p.nbocc; }
+
+// Pair-tile schedule: the tiles (bi <= bj) of every trajectory, trajectories
+// bin-packed over the 8 XCDs by their tiles' cost (an off-diagonal tile is 64
+// passes, a diagonal one 34), each XCD's tiles costliest first, queues
+// interleaved so that entry b runs on XCD b % 8.
+void build_tile_schedule(const tdhf_batch* b, std::vector<int4>& map) {
+  constexpr int kXcd = 8;
+  auto nblk = [&](int t) { return (b->p.nb * b->nk[t] + tdhf::kTileRows - 1) / tdhf::kTileRows; };
+  auto cost = [&](int t) {
+    const int n = nblk(t);
+    return 64.0 * n * (n - 1) / 2 + 34.0 * n;
+  };
+  std::vector<int> order(b->ntraj);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+  std::vector<double> load(kXcd, 0.0);
+  std::vector<std::vector<int4>> q(kXcd);
+  for (int t : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += cost(t);
+    const int n = nblk(t);
+    for (int bi = 0; bi < n; ++bi)
+      for (int bj = bi; bj < n; ++bj) q[x].push_back(make_int4(t, bi, bj, 0));
+  }
+  size_t longest = 0;
+  for (auto& v : q) {
+    std::stable_sort(v.begin(), v.end(),
+                     [](const int4& x, const int4& y) { return (x.y != x.z) > (y.y != y.z); });
+    longest = std::max(longest, v.size());
+  }
+  map.clear();
+  for (size_t i = 0; i < longest; ++i)
+    for (int x = 0; x < kXcd; ++x) map.push_back(i < q[x].size() ? q[x][i] : make_int4(-1, 0, 0, 0));
+  while (!map.empty() && map.back().x < 0) map.pop_back();
+}
+
+int batch_update_desc(tdhf_batch* b);
+
+// Switches the exchange between the row-wise kernel (0) and the pair tiles
+// (1): partial buffers and the tile schedule are set up on first use.
+int batch_set_xmode(tdhf_batch* b, int xmode) {
+  if (xmode && !batch_pairtile_ok(b))
+    return bfail(b, TDHF_EINVAL, "pair tiles need NG = 64 and NB == NBocc");
+  if (xmode && !b->d_xpart) {
+    b->xoff.resize(b->ntraj);
+    b->doff.resize(b->ntraj);
+    size_t xs = 0, ds = 0;
+    for (int t = 0; t < b->ntraj; ++t) {
+      const int rows = b->p.nb * b->nk[t];
+      const int nblk = (rows + tdhf::kTileRows - 1) / tdhf::kTileRows;
+      b->xoff[t] = xs;
+      b->doff[t] = ds;
+      xs += (size_t)rows * nblk * b->p.ng;
+      ds += (size_t)nblk * b->p.ng;
+    }
+    CHKRET(balloc(b, &b->d_xpart, xs));
+    CHKRET(balloc(b, &b->d_dpart, ds));
+    std::vector<int4> map;
+    build_tile_schedule(b, map);
+    b->ntiles = (int)map.size();
+    CHKRET(balloc(b, &b->d_tiles, map.size()));
+    BHIP(b, hipMemcpyAsync(b->d_tiles, map.data(), map.size() * sizeof(int4),
+                           hipMemcpyHostToDevice, b->stream));
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)tdhf::pairtile_lds_bytes()));
+  }
+  b->xmode = xmode;
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  return batch_update_desc(b);
+}
+
 int batch_update_desc(tdhf_batch* b) {
   for (int t = 0; t < b->ntraj; ++t) {
     tdhf::TrajDesc& d = b->h_td[t];
@@ -180,6 +264,10 @@ int batch_update_desc(tdhf_batch* b) {
     d.A = b->d_A ? b->d_A + (size_t)b->A_cap * t : nullptr;
     d.Aave = b->d_Aave ? b->d_Aave + (size_t)b->A_cap * t : nullptr;
     d.obs = b->d_obs ? b->d_obs + b->obs_off[t] * (size_t)b->obs_cap : nullptr;
+    const int rows = b->p.nb * b->nk[t];
+    d.nblk = (rows + tdhf::kTileRows - 1) / tdhf::kTileRows;
+    d.xpart = b->xmode ? b->d_xpart + b->xoff[t] : nullptr;
+    d.dpart = b->xmode ? b->d_dpart + b->doff[t] : nullptr;
   }
   BHIP(b, hipMemcpyAsync(b->d_td, b->h_td.data(), b->h_td.size() * sizeof(tdhf::TrajDesc),
                          hipMemcpyHostToDevice, b->stream));
@@ -188,6 +276,7 @@ int batch_update_desc(tdhf_batch* b) {
 }
 
 int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
+  // with timing on, one event pair brackets the stage (pair tiles + fused kernel)
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (b->timing) {
     if (b->ev_used + 2 > b->ev.size())
@@ -199,6 +288,17 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
     e0 = b->ev[b->ev_used++];
     e1 = b->ev[b->ev_used++];
     BHIP(b, hipEventRecord(e0, b->stream));
+  }
+  if (b->xmode) {
+    tdhf::PairTileArgs pa{};
+    pa.td = b->d_td;
+    pa.tiles = b->d_tiles;
+    pa.tw = a.tw;
+    pa.nb = b->p.nb;
+    pa.mode = a.mode;
+    hipLaunchKernelGGL(tdhf::pairtile_kernel, dim3(b->ntiles), dim3(512),
+                       tdhf::pairtile_lds_bytes(), b->stream, pa);
+    BHIP(b, hipGetLastError());
   }
   const size_t lds = lds_for(b->p.ng, nw);
   switch (b->p.ng) {
@@ -255,7 +355,8 @@ int batch_enqueue_steps(tdhf_batch* b, int n, int nw) {
 int batch_run_steps(tdhf_batch* b, int nt) {
   int done = 0;
   if (!b->timing && nt >= kGraphSteps) {
-    if (b->gexec && (b->graph_nw != b->nw || b->graph_sched != b->sched)) {
+    if (b->gexec && (b->graph_nw != b->nw || b->graph_sched != b->sched ||
+                     b->graph_xmode != b->xmode)) {
       (void)hipGraphExecDestroy(b->gexec);
       b->gexec = nullptr;
     }
@@ -278,6 +379,7 @@ int batch_run_steps(tdhf_batch* b, int nt) {
       b->graph_steps = kGraphSteps;
       b->graph_nw = b->nw;
       b->graph_sched = b->sched;
+      b->graph_xmode = b->xmode;
     }
     while (nt - done >= b->graph_steps) {
       BHIP(b, hipGraphLaunch(b->gexec, b->stream));
@@ -323,15 +425,16 @@ void batch_release(tdhf_batch* b) {
   b->ev.clear();
   dfree(b->d_vion); dfree(b->d_state); dfree(b->d_td); dfree(b->d_rowmap);
   dfree(b->d_A); dfree(b->d_Aave); dfree(b->d_obs); dfree(b->d_res); dfree(b->d_step);
+  dfree(b->d_xpart); dfree(b->d_dpart); dfree(b->d_tiles);
   for (auto c : b->tabs) tdhf_destroy(c);
   b->tabs.clear();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   b->stream = nullptr;
 }
 
-// autotuner cache: shape -> (waves per workgroup, row schedule)
-std::map<std::vector<int>, std::pair<int, int>>& tune_cache() {
-  static std::map<std::vector<int>, std::pair<int, int>> m;
+// autotuner cache: shape -> (waves per workgroup, row schedule, exchange mode)
+std::map<std::vector<int>, std::array<int, 3>>& tune_cache() {
+  static std::map<std::vector<int>, std::array<int, 3>> m;
   return m;
 }
 
@@ -414,12 +517,13 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
   // groups of a single wave need no workgroup barrier, and eight such
   // workgroups share a CU) and the XCD-pinned schedule
   auto ct = tune_cache().find(tune_key(b));
-  b->nw = ct != tune_cache().end() ? ct->second.first : (p->ng + 63) / 64;
+  const bool cached = ct != tune_cache().end();
+  b->nw = cached ? ct->second[0] : (p->ng + 63) / 64;
   b->ngrid = 0;
-  if ((rc = batch_set_schedule(b, ct != tune_cache().end() ? ct->second.second : 0)))
-    return bail(rc);
+  if ((rc = batch_set_schedule(b, cached ? ct->second[1] : 0))) return bail(rc);
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
+  if (cached && ct->second[2] && (rc = batch_set_xmode(b, 1))) return bail(rc);
   {
     const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
                      : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
@@ -538,6 +642,13 @@ int tdhf_batch_set_schedule(tdhf_batch* b, int sched) {
 
 int tdhf_batch_get_schedule(const tdhf_batch* b) { return b ? b->sched : -1; }
 
+int tdhf_batch_set_exchange(tdhf_batch* b, int mode) {
+  if (!b || mode < 0 || mode > 1) return bfail(b, TDHF_EINVAL, "invalid exchange mode %d", mode);
+  return batch_set_xmode(b, mode);
+}
+
+int tdhf_batch_get_exchange(const tdhf_batch* b) { return b ? b->xmode : -1; }
+
 int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const double* Aave,
                         int* best_nw) {
   if (!b || nt_trial < 1 || !A || !Aave) return bfail(b, TDHF_EINVAL, "invalid autotune arguments");
@@ -549,10 +660,12 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
     ~FreeOnReturn() { dfree(p); }
   } save_guard{save};
   BHIP(b, hipMemcpyAsync(save, b->d_state, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
-  int best = b->nw, best_s = b->sched;
+  int best = b->nw, best_s = b->sched, best_x = b->xmode;
   double best_t = 1e300;
   const int cand[4] = {1, 2, 4, 8};
+  for (int xm = 0; xm <= (batch_pairtile_ok(b) ? 1 : 0); ++xm)
   for (int sc = 0; sc < kNumSchedules; ++sc) {
+    CHKRET(batch_set_xmode(b, xm));
     CHKRET(batch_set_schedule(b, sc));
     for (int nw : cand) {
       if (!batch_valid_nw(b, nw)) continue;
@@ -575,14 +688,16 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
         best_t = dt_ms;
         best = nw;
         best_s = sc;
+        best_x = xm;
       }
     }
   }
   BHIP(b, hipMemcpyAsync(b->d_state, save, b->total * sizeof(zd), hipMemcpyDeviceToDevice, b->stream));
   BHIP(b, hipStreamSynchronize(b->stream));
   b->nw = best;
+  CHKRET(batch_set_xmode(b, best_x));
   CHKRET(batch_set_schedule(b, best_s));
-  tune_cache()[tune_key(b)] = std::make_pair(best, best_s);
+  tune_cache()[tune_key(b)] = {best, best_s, best_x};
   if (best_nw) *best_nw = best;
   return 0;
 }

@@ -53,6 +53,12 @@ struct TrajDesc {
   const double* A;
   const double* Aave;
   double* obs;
+  // exchange from the pair tiles (tdhf_pairtile.h): nblk partial sums per
+  // row (row r, block c at (r * nblk + c) * NG) and nblk density partials
+  // (block b at b * NG); nullptr: the row-wise exchange of apply_kernel
+  zd* xpart;
+  double* dpart;
+  int nblk;
 };
 
 // Offsets inside one observables slot: rows*3 partials then (N, Eion, EH).
@@ -233,6 +239,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const zd *u = a.u, *w = a.w;
   zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext, *out = a.out;
   double* obs = a.obs;
+  const zd* xpart = nullptr;        // exchange partials of the pair tiles
+  const double* dpart = nullptr;
+  int nblk = 0;
   const int tr = blockIdx.y;  // trajectory of a uniform batch (0 otherwise)
   if (tr > 0) {
     Aarr += (size_t)tr * a.fstride;
@@ -255,6 +264,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     u0 = t.u0;
     ksum = t.ksum;
     obs = t.obs;
+    xpart = t.xpart;
+    dpart = t.dpart;
+    nblk = t.nblk;
     switch (a.mode) {
       case M_S1: u = t.u0; unext = t.sa; break;
       case M_S2: u = t.sa; unext = t.sb; break;
@@ -307,7 +319,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // which go to LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   F fft;
-  fft.init(a.tw, l);
+  if (!xpart) fft.init(a.tw, l);
 
   zd acc[R], un[R];
   double den[R];
@@ -316,7 +328,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // u(q,j) of the next pair is fetched while the current pair is transformed;
   // the multipliers of the current pair are loaded before that prefetch, so
   // waiting for them leaves the prefetch in flight.
-  const int nitem = nk * a.nbocc;
+  const int nitem = xpart ? 0 : nk * a.nbocc;  // pair tiles: no pairs here
   // item = jb + nbocc iq; a group's items are ngrp apart
   const int dq = ngrp / a.nbocc, dj = ngrp - dq * a.nbocc;
   int cq = grp / a.nbocc, cj = grp - cq * a.nbocc;  // item of the current pass
@@ -442,6 +454,17 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
+  if (xpart) {
+    // exchange and density from the pair tiles' partials, added in block order
+    if (tid < NG) {
+      zd vf = mk(0.0, 0.0);
+      for (int c = 0; c < nblk; ++c) vf = zadd(vf, gload(xpart + ((size_t)row * nblk + c) * NG + tid));
+      double rho = 0.0;
+      for (int c = 0; c < nblk; ++c) rho += gload(dpart + (size_t)c * NG + tid);
+      sVF[tid] = vf;
+      sRho[tid] = rho;
+    }
+  } else
   // ---- one reduction over all groups (fixed order) of V_F w and the density
   // sum; the sums land in sVF and, right after it, sRho (sum |u|^2, without
   // occ) ----

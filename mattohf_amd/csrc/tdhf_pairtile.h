@@ -1,0 +1,235 @@
+// tdhf_pairtile.h -- exchange of batched RK4 stages with the pair symmetry
+// (gfx950, FP64; NG = 64, NB == NBocc).
+//
+// In an RK4 stage the exchange acts on the state itself (w = u in
+// ubkwbk_vFwbk, src/Functions.f90:237-261), so the pair convolutions of
+// rows r = (k,i) and s = (q,j),
+//     c_rs = IFFT[ t_kq(G) FFT[u_r conj(u_s)] ]      (t: the scaled table
+//                                                      -v(G+k-q)/(NG NK)),
+// come in conjugate pairs: v is even, so t_qk(G) = t_kq(-G) for every grid G
+// except the Nyquist one (the grid has -NG/2 b but not +NG/2 b), and
+//     c_sr(x) = conj(c_rs(x)) + D_kq conj(P(NG/2)) (-1)^x,
+//     D_kq = t_qk(NG/2) - t_kq(NG/2),  P = FFT[u_r conj(u_s)].
+// One transform pair therefore serves row r (c_rs u_s) and row s (c_sr u_r).
+//
+// pairtile_kernel computes, for one trajectory and one tile of 64 x 64 rows
+// (row blocks bi <= bj), every pair once and writes per-row partial sums;
+// the fused apply kernel then adds a row's partials (one per row block, in
+// block order) and continues with the Hartree and kinetic terms and the
+// stage update as for the row-wise exchange.
+//
+// Tile schedule (checked on the host for coverage): group g = 8 w + gw
+// (wave w, group gw of the wave) owns row r = 64 bi + g and keeps its sum in
+// registers.  In pass (p1, p0) it pairs r with s = 64 bj + 8 ((w + p1) % 8)
+// + (gw + p0) % 8, whose sum lives in LDS.  For a fixed p1 (a window of
+// eight passes) wave w alone touches the LDS rows of block (w + p1) % 8, in
+// program order, so the LDS sums need no atomics and their order is fixed;
+// a workgroup barrier separates the windows.  Off-diagonal tiles (bi < bj)
+// run all 64 offsets two-sided.  Diagonal tiles run the offsets o with
+// o < -o (mod 8 in each coordinate) two-sided and the four self-inverse ones
+// ((0,0): the self pairs; (0,4), (4,0), (4,4)) one-sided: 34 passes cover
+// all 64 x 64 ordered pairs.
+#pragma once
+#include "tdhf_kernels.h"
+
+namespace tdhf {
+
+// One tile of the pair schedule: trajectory (x), row blocks bi (y) <= bj (z);
+// x < 0 marks a padding entry.
+struct PairTileArgs {
+  const TrajDesc* td;
+  const int4* tiles;
+  const zd* tw;  // twiddle table of GroupFFT<64>
+  int nb;        // NB (== NBocc)
+  int mode;      // RK4 stage of the launch (selects the input state)
+};
+
+constexpr int kTileRows = 64;
+constexpr int kTilePitch = 72;  // complex elements per LDS row of the tile sums
+
+__host__ __device__ constexpr size_t pairtile_lds_bytes() {
+  // 64 transpose tiles (one per group) and 64 row sums, both 72 complex
+  return (size_t)2 * 64 * kTilePitch * sizeof(zd);
+}
+
+// The passes of a diagonal tile: offsets (p1, p0) with o <= -o.
+__device__ __forceinline__ bool diag_pass(int p1, int p0, bool* both) {
+  const int m1 = (8 - p1) & 7, m0 = (8 - p0) & 7;
+  if (p1 == m1 && p0 == m0) {
+    *both = false;
+    return true;
+  }
+  *both = true;
+  return p1 < m1 || (p1 == m1 && p0 < m0);
+}
+
+__global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
+  constexpr int NG = 64;
+  using F = GroupFFT<NG>;
+  constexpr int R = F::R, L = F::L;  // 8 x 8
+  constexpr int JNYQ = F::NYQ_REG;   // lane 0 of a group holds G index NG/2 here
+  static_assert(F::TSZ <= kTilePitch, "transpose tile exceeds its LDS slot");
+  extern __shared__ zd smem[];
+  const int4 tile = a.tiles[blockIdx.x];
+  if (tile.x < 0) return;  // padding entry (whole workgroup)
+  const TrajDesc& t = a.td[tile.x];
+  const int bi = tile.y, bj = tile.z;
+  const bool diag = bi == bj;
+  const int nb = a.nb, nk = t.nk, rows = nb * nk;
+  const zd* u;
+  switch (a.mode) {
+    case M_S2: u = t.sa; break;
+    case M_S3: u = t.sb; break;
+    case M_S4: u = t.sa; break;
+    default: u = t.u0; break;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l = lane % L, gw = lane / L;
+  const int g = wave * 8 + gw;
+  zd* T = smem + g * kTilePitch;           // this group's transpose tile
+  zd* acj = smem + 64 * kTilePitch;        // row sums of the J block
+  const int r = kTileRows * bi + g;
+  const bool rv = r < rows;
+  const int rc = rv ? r : rows - 1;        // clamped for table addresses
+  const int kr = rc / nb;
+  const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = 8 m + l
+
+  F fft;
+  fft.init(a.tw, l);
+  zd ur[R], acc[R], un[R];
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    ur[m] = rv ? gload(u + (size_t)NG * r + L * m + l) : mk(0.0, 0.0);
+    acc[m] = mk(0.0, 0.0);
+  });
+  // zero the J sums
+  for (int i = tid; i < 64 * kTilePitch; i += blockDim.x) acj[i] = mk(0.0, 0.0);
+
+  // pass list: off-diagonal 64 offsets, diagonal the 34 of diag_pass
+  auto partner = [&](int p1, int p0) {
+    return kTileRows * bj + 8 * ((wave + p1) & 7) + ((gw + p0) & 7);
+  };
+  auto next_pass = [&](int q) {  // next pass index after q that runs, or 64
+    for (++q; q < 64; ++q) {
+      bool two;
+      if (!diag || diag_pass(q >> 3, q & 7, &two)) break;
+    }
+    return q;
+  };
+  auto fetch = [&](int s) {
+    const bool sv = s < rows;
+    const zd* urow = u + (size_t)NG * (sv ? s : 0);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      un[m] = sv ? gload(urow + L * m + l) : mk(0.0, 0.0);
+    });
+  };
+  int p = 0;  // pass 0 (offset (0,0)) runs in both kinds of tile
+  fetch(partner(0, 0));
+  lds_barrier();  // J sums zeroed
+
+  for (int p1 = 0; p1 < 8; ++p1) {
+    for (; p < 64 && (p >> 3) == p1; p = next_pass(p)) {
+      const int p0 = p & 7;
+      bool both = true;
+      if (diag) diag_pass(p1, p0, &both);
+      const int s = partner(p1, p0);
+      const int sc = s < rows ? s : rows - 1;
+      const int ks = sc / nb;
+      zd us[R], v[R];
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        us[m] = un[m];
+      });
+      // the Coulomb row of (k_r, k_s) and the Nyquist difference D
+      const double* vt = t.vtl + (size_t)(kr - ks + nk - 1) * NG + l * R;
+      double vtr[R];
+      static_for<0, R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        vtr[j] = gload(vt + j);
+      });
+      const double dnyq = gload(t.vtl + (size_t)(ks - kr + nk - 1) * NG + JNYQ) -
+                          gload(t.vtl + (size_t)(kr - ks + nk - 1) * NG + JNYQ);
+      // prefetch the partner of the next pass (the last pass re-reads its own)
+      const int pn = next_pass(p);
+      fetch(pn < 64 ? partner(pn >> 3, pn & 7) : s);
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        v[m] = zmulc(ur[m], us[m]);
+      });
+      fft.fwd(v, T, l);
+      const double nx = __shfl(v[JNYQ].x, gw * L, 64);
+      const double ny = __shfl(v[JNYQ].y, gw * L, 64);
+      static_for<0, R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = zscal(v[j], vtr[j]);
+      });
+      fft.inv(v, T, l);
+      // row r: c_rs u_s
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        acc[m].x = fma(v[m].x, us[m].x, acc[m].x);
+        acc[m].x = fma(-v[m].y, us[m].y, acc[m].x);
+        acc[m].y = fma(v[m].x, us[m].y, acc[m].y);
+        acc[m].y = fma(v[m].y, us[m].x, acc[m].y);
+      });
+      if (both) {
+        // row s: (conj(c_rs) + D conj(P(NG/2)) (-1)^x) u_r, into its LDS sum
+        const double fx = sgn * dnyq * nx, fy = -sgn * dnyq * ny;
+        zd* js = acj + (size_t)(s - kTileRows * bj) * kTilePitch;
+        static_for<0, R>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          const double cx = v[m].x + fx, cy = -v[m].y + fy;
+          zd sm = js[L * m + l];
+          sm.x = fma(cx, ur[m].x, sm.x);
+          sm.x = fma(-cy, ur[m].y, sm.x);
+          sm.y = fma(cx, ur[m].y, sm.y);
+          sm.y = fma(cy, ur[m].x, sm.y);
+          js[L * m + l] = sm;
+        });
+      }
+    }
+    lds_barrier();  // the next window hands the J blocks to other waves
+  }
+
+  // ---- per-row partials: slot c of row r holds its pairs with block c ----
+  const int nblk = t.nblk;
+  if (rv) {
+    zd* out = t.xpart + ((size_t)r * nblk + bj) * NG;
+    const zd* own = acj + (size_t)g * kTilePitch;  // diagonal: this row's J sum
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      zd val = acc[m];
+      if (diag) val = zadd(val, own[L * m + l]);
+      gstore(out + L * m + l, val);
+    });
+  }
+  if (!diag) {
+    for (int i = tid; i < 64 * NG; i += blockDim.x) {
+      const int j = i / NG, x = i - j * NG;
+      const int s = kTileRows * bj + j;
+      if (s < rows) gstore(t.xpart + ((size_t)s * nblk + bi) * NG + x, acj[(size_t)j * kTilePitch + x]);
+    }
+  } else {
+    // density partial of the block: sum over its rows of |u_r|^2 (without
+    // the occupation), first over the groups of a wave in registers, then
+    // over the waves through LDS (the transpose tiles are free now)
+    double den[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      den[m] = zabs2(ur[m]);
+    });
+    wave_reduce_scatter<L, R>(den);  // lane (gw, l) holds point x = 8 gw + l
+    lds_barrier();                   // everyone has read its J sum above
+    double* wp = reinterpret_cast<double*>(smem);
+    wp[wave * NG + L * gw + l] = den[0];
+    lds_barrier();
+    if (tid < NG) {
+      double sden = 0.0;
+      for (int w2 = 0; w2 < (int)(blockDim.x >> 6); ++w2) sden += wp[w2 * NG + tid];
+      gstore(t.dpart + (size_t)bi * NG + tid, sden);
+    }
+  }
+}
+
+}  // namespace tdhf

@@ -84,6 +84,8 @@ SIGNATURES = {
     "tdhf_batch_autotune": (_i, [_vp, _i, _vp, _vp, C.POINTER(_i)]),
     "tdhf_batch_set_schedule": (_i, [_vp, _i]),
     "tdhf_batch_get_schedule": (_i, [_vp]),
+    "tdhf_batch_set_exchange": (_i, [_vp, _i]),
+    "tdhf_batch_get_exchange": (_i, [_vp]),
     "tdhf_batch_set_kernel_timing": (_i, [_vp, _i]),
     "tdhf_batch_kernel_stats": (_i, [_vp, C.POINTER(C.c_long), C.POINTER(_d)]),
 }
@@ -539,6 +541,13 @@ class Batch:
 
     def schedule(self):
         return load().tdhf_batch_get_schedule(self.h)
+
+    def set_exchange(self, mode):
+        """0: row-wise exchange; 1: pair tiles (NG=64, NB == NBocc)."""
+        _bchk(load().tdhf_batch_set_exchange(self.h, int(mode)), self.h)
+
+    def exchange(self):
+        return load().tdhf_batch_get_exchange(self.h)
 
     def autotune(self, A, Aave):
         A = np.ascontiguousarray(A, dtype=np.float64)

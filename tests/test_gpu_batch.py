@@ -78,9 +78,39 @@ def test_autotune_keeps_state(batch_case):
         best = B.autotune(A[:, :30], Aave[:, :30])
         assert best in (1, 2, 4, 8) and B.waves() == best
         assert B.schedule() in (0, 1, 2)
+        assert B.exchange() == 0  # NG = 32: the pair tiles are for NG = 64
         after = B.download()
         for a, b in zip(after, us):
             assert np.array_equal(a, b)
+
+
+def test_autotune_ng64_keeps_state_and_results():
+    """NG=64 batches: the autotuner also tries the pair-tile exchange; the
+    chosen configuration reproduces the single runs."""
+    ng, nb, a, dt, nt, nks = 64, 2, 10.0, 0.01, 30, [4, 8, 4]
+    us, vions, fields = [], [], []
+    for t, nk in enumerate(nks):
+        S = po.System(ng, nk, nb, nb, a)
+        rng = np.random.default_rng(700 + t)
+        u = rng.standard_normal(S.shape) + 1j * rng.standard_normal(S.shape)
+        us.append(S.gram_schmidt(np.asfortranarray(u)))
+        vions.append(S.ion_potential([8.0], [1.0], [0.0]))
+        _, A, Aave, _ = T.define_afield_lp(nt, dt, *T.pulse_lp_params(0.1 + 0.1 * t, 0.057, 0.3))
+        fields.append((A, Aave))
+    A = np.array([f[0] for f in fields])
+    Aave = np.array([f[1] for f in fields])
+    with T.Batch(ng, nb, nb, a, dt, nks, np.array(vions)) as B:
+        B.upload(us)
+        B.autotune(A[:, :25], Aave[:, :25])
+        assert B.exchange() in (0, 1)
+        for x, y in zip(B.download(), us):
+            assert np.array_equal(x, y)
+        e, n, j, i3 = B.propagate(A, Aave)
+    for t, nk in enumerate(nks):
+        with T.TDHF(ng, nk, nb, nb, a, dt, vions[t]) as G:
+            u1, e1, n1, j1, _ = G.RT_calculation(us[t], A[t], Aave[t])
+        np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * max(1.0, np.abs(e1).max()))
+        np.testing.assert_allclose(j[t], j1, rtol=0, atol=1e-12 * max(1.0, np.abs(j1).max()))
 
 
 @pytest.mark.parametrize("nw", [1, 2])

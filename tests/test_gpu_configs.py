@@ -191,6 +191,60 @@ def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
         np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
 
 
+def test_cfg3_pair_tiles_equal_single_runs(cfg3):
+    """The exchange from pair tiles (one transform pair per unordered pair of
+    rows, Nyquist correction for the reversed pair) against the row-wise
+    single runs, for NK = 16, 32, 64 (one, two, four 64-row blocks)."""
+    systems, us, vions, A, Aave, singles = cfg3
+    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+        B.set_exchange(1)
+        assert B.exchange() == 1
+        B.upload(us)
+        e, n, j, i3 = B.propagate(A, Aave)
+        finals = B.download()
+    for t in range(len(C3["nks"])):
+        u1, e1, n1, j1, i31 = singles[t]
+        np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * np.abs(e1).max())
+        np.testing.assert_allclose(n[t], n1, rtol=0, atol=1e-12)
+        np.testing.assert_allclose(j[t], j1, rtol=0, atol=1e-12 * max(1.0, np.abs(j1).max()))
+        np.testing.assert_allclose(i3[t], i31, rtol=0, atol=1e-12 * np.abs(i31).max())
+        np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
+
+
+def test_pair_tiles_partial_blocks_match_oracle():
+    """Row counts that are not multiples of the 64-row block (NK = 10: 40
+    rows; NK = 20: 80 rows; NK = 3: 12 rows), against the oracle."""
+    ng, nb, a, dt, nt, nks = 64, 4, 10.0, 0.01, 8, [10, 20, 3]
+    us, vions, systems = [], [], []
+    for t, nk in enumerate(nks):
+        S = po.System(ng, nk, nb, nb, a)
+        systems.append(S)
+        us.append(random_orbitals(S, 600 + t))
+        vions.append(S.ion_potential([8.0], [1.0], [0.0]))
+    A, Aave = seeded_pulses(len(nks), 20000 + nt, dt, 400.0, 20261206)
+    A, Aave = np.ascontiguousarray(A[:, 20000:]), np.ascontiguousarray(Aave[:, 20000:])
+    with T.Batch(ng, nb, nb, a, dt, nks, np.array(vions)) as B:
+        B.set_exchange(1)
+        B.upload(us)
+        e, n, j, _ = B.propagate(A, Aave)
+        finals = B.download()
+    for t in range(len(nks)):
+        O = po.Oracle(systems[t], vions[t])
+        uo, eo, no, jo, _ = O.RT_calculation(us[t], dt, A[t], Aave[t])
+        assert np.abs(e[t] - eo).max() < 1e-10
+        assert np.abs(n[t] - no).max() < 1e-10
+        assert rel(j[t], jo) < 1e-8
+        assert rel(finals[t], uo) < 1e-10
+
+
+def test_pair_tiles_need_full_occupation():
+    S = po.System(64, 4, 3, 2, 10.0)
+    with T.Batch(64, 3, 2, 10.0, 0.01, [4], np.array([np.zeros(64)])) as B:
+        with pytest.raises(T.TDHFError):
+            B.set_exchange(1)
+        assert B.exchange() == 0
+
+
 def test_cfg3_batch_matches_oracle(cfg3):
     """One trajectory of each k-mesh of the batch against the oracle, over an
     8-step prefix."""
