@@ -208,8 +208,13 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
         "row_schedule": sched,
         "exchange": "pair tiles" if xmode == 1 else "row-wise",
         "avg_launch_us": avg_s * 1e6,
+        "avg_launch_note": "HIP events around one RK4 stage of the whole batch (with pair "
+                           "tiles: the pair-tile kernel and the fused stage kernel together)",
         "achieved_tflops": f_launch / avg_s / 1e12,
         "frac": f_launch / avg_s / 1e12 / PEAK_FP64_TFLOPS,
+        "flop_basis": "the reference algorithm's count (all NK^2 NB NBocc ordered exchange "
+                      "pairs per H application); pair tiles run one transform pair per "
+                      "unordered pair, so they execute fewer FP64 operations than this basis",
         "observables_finite": bool(np.isfinite(e).all() and np.isfinite(j).all()),
     }
 

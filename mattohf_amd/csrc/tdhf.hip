@@ -182,16 +182,18 @@ void dfree(T*& p) {
 
 // Apply-kernel dispatch over the supported grid sizes.
 template <int NG>
-size_t lds_bytes(int nw) {
-  return tdhf::apply_lds_bytes<NG>(nw);
+size_t lds_bytes(int nw, bool epilogue_only) {
+  return tdhf::apply_lds_bytes<NG>(nw, epilogue_only);
 }
 
-size_t lds_for(int ng, int nw) {
+// LDS of an apply_kernel launch; epilogue_only: the exchange comes from the
+// pair tiles (no transpose tiles needed)
+size_t lds_for(int ng, int nw, bool epilogue_only = false) {
   switch (ng) {
-    case 16: return lds_bytes<16>(nw);
-    case 32: return lds_bytes<32>(nw);
-    case 64: return lds_bytes<64>(nw);
-    case 128: return lds_bytes<128>(nw);
+    case 16: return lds_bytes<16>(nw, epilogue_only);
+    case 32: return lds_bytes<32>(nw, epilogue_only);
+    case 64: return lds_bytes<64>(nw, epilogue_only);
+    case 128: return lds_bytes<128>(nw, epilogue_only);
   }
   return 0;
 }

@@ -300,7 +300,7 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
                        tdhf::pairtile_lds_bytes(), b->stream, pa);
     BHIP(b, hipGetLastError());
   }
-  const size_t lds = lds_for(b->p.ng, nw);
+  const size_t lds = lds_for(b->p.ng, nw, b->xmode != 0);
   switch (b->p.ng) {
     case 16: launch_apply_t<16>(a, b->ngrid, 1, nw, lds, b->stream); break;
     case 32: launch_apply_t<32>(a, b->ngrid, 1, nw, lds, b->stream); break;

@@ -203,13 +203,23 @@ __host__ __device__ constexpr int apply_slot() {
   return need + ((8 - need % 16) + 16) % 16;
 }
 
+// Slot of a launch that takes its exchange from the pair tiles (no pair loop,
+// no transposes): just enough for the epilogue scratch at the smallest launch.
+template <int NG>
+__host__ __device__ constexpr int apply_epi_slot() {
+  constexpr int nwmin = NG > 64 ? NG / 64 : 1, gpw = 64 / GroupFFT<NG>::L;
+  constexpr int need = (5 * 64 * nwmin + 7 * NG + 2 * gpw * nwmin - 1) / (2 * gpw * nwmin);
+  return need + ((8 - need % 16) + 16) % 16;
+}
+
 // LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' slots
 // (also the circulant-product partials and observables staging after the
 // pair loop), then w, V_F w (followed by rho), K_1 (complex NG each), rho,
 // K_H, K_2 (NG doubles each), a small scratch and the reduction scratch.
 template <int NG>
-__host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
-  return (size_t)nw * (64 / GroupFFT<NG>::L) * apply_slot<NG>() * sizeof(zd) +
+__host__ __device__ constexpr size_t apply_lds_bytes(int nw, bool epilogue_only = false) {
+  return (size_t)nw * (64 / GroupFFT<NG>::L) *
+             (epilogue_only ? apply_epi_slot<NG>() : apply_slot<NG>()) * sizeof(zd) +
          3 * NG * sizeof(zd) + 3 * NG * sizeof(double) + 16 * sizeof(double) +
          apply_part_doubles(nw, NG) * sizeof(double);
 }
@@ -225,6 +235,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
                     2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
                 "slot area too small for the epilogue scratch");
+  static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
+                    2 * apply_epi_slot<NG>() * GPW * (NG > 64 ? NG / 64 : 1),
+                "epilogue-only slot too small for the epilogue scratch");
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -278,8 +291,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   }
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
-  zd* T = smem + grp * SSZ;
-  zd* sW = smem + ngrp * SSZ;
+  // exchange from the pair tiles: no transposes, only the epilogue scratch
+  const int ssz = xpart ? apply_epi_slot<NG>() : SSZ;
+  zd* T = smem + grp * ssz;
+  zd* sW = smem + ngrp * ssz;
   zd* sVF = sW + NG;
   double* sRho = reinterpret_cast<double*>(sVF + NG);  // follows V_F w (one reduction)
   zd* sK1 = reinterpret_cast<zd*>(sRho + NG);
@@ -346,6 +361,16 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     w_o = gload(w + o);
     k2_o = gload(kc + NG + tid);
     k1_o = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
+  }
+  // exchange and density from the pair tiles' partials, added in block order
+  // (loaded with the other operands; used after the empty pair loop)
+  zd xvf = mk(0.0, 0.0);
+  double xrho = 0.0;
+  if (xpart && tid < NG) {
+    for (int c = 0; c < nblk; ++c) {
+      xvf = zadd(xvf, gload(xpart + ((size_t)row * nblk + c) * NG + tid));
+      xrho += gload(dpart + (size_t)c * NG + tid);
+    }
   }
   const double kxv = gload(kx + ik);
   static_for<0, R>([&](auto mc) {
@@ -455,14 +480,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   }
 
   if (xpart) {
-    // exchange and density from the pair tiles' partials, added in block order
     if (tid < NG) {
-      zd vf = mk(0.0, 0.0);
-      for (int c = 0; c < nblk; ++c) vf = zadd(vf, gload(xpart + ((size_t)row * nblk + c) * NG + tid));
-      double rho = 0.0;
-      for (int c = 0; c < nblk; ++c) rho += gload(dpart + (size_t)c * NG + tid);
-      sVF[tid] = vf;
-      sRho[tid] = rho;
+      sVF[tid] = xvf;
+      sRho[tid] = xrho;
     }
   } else
   // ---- one reduction over all groups (fixed order) of V_F w and the density

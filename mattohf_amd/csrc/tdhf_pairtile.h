@@ -18,7 +18,7 @@
 // block order) and continues with the Hartree and kinetic terms and the
 // stage update as for the row-wise exchange.
 //
-// Tile schedule (checked on the host for coverage): group g = 8 w + gw
+// Tile schedule (every ordered pair of a tile exactly once): group g = 8 w + gw
 // (wave w, group gw of the wave) owns row r = 64 bi + g and keeps its sum in
 // registers.  In pass (p1, p0) it pairs r with s = 64 bj + 8 ((w + p1) % 8)
 // + (gw + p0) % 8, whose sum lives in LDS.  For a fixed p1 (a window of
