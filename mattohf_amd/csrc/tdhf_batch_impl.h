@@ -56,6 +56,7 @@ struct tdhf_batch {
   int4* d_tiles = nullptr;
   int ntiles = 0;
   std::vector<size_t> xoff, doff;  // per trajectory (complex / double elements)
+  size_t zoff = 0;                 // NG zeros in d_dpart after the partials
   bool timing = false;
   long launches = 0;
   double total_ms = 0.0;
@@ -229,7 +230,10 @@ int batch_set_xmode(tdhf_batch* b, int xmode) {
       ds += (size_t)nblk * b->p.ng;
     }
     CHKRET(balloc(b, &b->d_xpart, xs));
-    CHKRET(balloc(b, &b->d_dpart, ds));
+    // the density partials, then NG zeros (the table row of absent partners)
+    CHKRET(balloc(b, &b->d_dpart, ds + b->p.ng));
+    b->zoff = ds;
+    BHIP(b, hipMemsetAsync(b->d_dpart + ds, 0, b->p.ng * sizeof(double), b->stream));
     std::vector<int4> map;
     build_tile_schedule(b, map);
     b->ntiles = (int)map.size();
@@ -294,6 +298,7 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
     pa.td = b->d_td;
     pa.tiles = b->d_tiles;
     pa.tw = a.tw;
+    pa.zrow = b->d_dpart + b->zoff;
     pa.nb = b->p.nb;
     pa.mode = a.mode;
     hipLaunchKernelGGL(tdhf::pairtile_kernel, dim3(b->ntiles), dim3(512),

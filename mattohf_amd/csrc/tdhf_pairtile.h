@@ -40,8 +40,9 @@ struct PairTileArgs {
   const TrajDesc* td;
   const int4* tiles;
   const zd* tw;  // twiddle table of GroupFFT<64>
-  int nb;        // NB (== NBocc)
-  int mode;      // RK4 stage of the launch (selects the input state)
+  const double* zrow;  // NG zeros: the table row used for partners past the last row
+  int nb;              // NB (== NBocc)
+  int mode;            // RK4 stage of the launch (selects the input state)
 };
 
 constexpr int kTileRows = 64;
@@ -52,16 +53,21 @@ __host__ __device__ constexpr size_t pairtile_lds_bytes() {
   return (size_t)2 * 64 * kTilePitch * sizeof(zd);
 }
 
-// The passes of a diagonal tile: offsets (p1, p0) with o <= -o.
-__device__ __forceinline__ bool diag_pass(int p1, int p0, bool* both) {
-  const int m1 = (8 - p1) & 7, m0 = (8 - p0) & 7;
-  if (p1 == m1 && p0 == m0) {
-    *both = false;
-    return true;
+// The passes of a diagonal tile, as bit masks over p = 8 p1 + p0: the offsets
+// with o <= -o run, two-sided unless o == -o.
+__host__ __device__ constexpr uint64_t diag_pass_mask(bool two_sided) {
+  uint64_t m = 0;
+  for (int p = 0; p < 64; ++p) {
+    const int p1 = p >> 3, p0 = p & 7, m1 = (8 - p1) & 7, m0 = (8 - p0) & 7;
+    const bool self = p1 == m1 && p0 == m0;
+    const bool run = self || p1 < m1 || (p1 == m1 && p0 < m0);
+    if (run && (!two_sided || !self)) m |= uint64_t(1) << p;
   }
-  *both = true;
-  return p1 < m1 || (p1 == m1 && p0 < m0);
+  return m;
 }
+constexpr uint64_t kDiagRun = diag_pass_mask(false), kDiagBoth = diag_pass_mask(true);
+static_assert(__builtin_popcountll(kDiagRun) == 34 && __builtin_popcountll(kDiagBoth) == 30,
+              "diagonal tiles: 34 passes, 30 of them two-sided");
 
 __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
   constexpr int NG = 64;
@@ -72,10 +78,15 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
   extern __shared__ zd smem[];
   const int4 tile = a.tiles[blockIdx.x];
   if (tile.x < 0) return;  // padding entry (whole workgroup)
-  const TrajDesc& t = a.td[tile.x];
+  const TrajDesc t = a.td[tile.x];  // by value: its fields stay in registers
   const int bi = tile.y, bj = tile.z;
   const bool diag = bi == bj;
   const int nb = a.nb, nk = t.nk, rows = nb * nk;
+  const double* vtl = t.vtl;
+  // k-point of a row, s / nb, by a float product: exact while s < 2^22
+  // ((s + 1/2)/nb lies at least 1/(2 nb) from an integer)
+  const float inb = 1.0f / (float)nb;
+  auto kpt = [&](int s) { return __float2int_rd(((float)s + 0.5f) * inb); };
   const zd* u;
   switch (a.mode) {
     case M_S2: u = t.sa; break;
@@ -91,12 +102,12 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
   const int r = kTileRows * bi + g;
   const bool rv = r < rows;
   const int rc = rv ? r : rows - 1;        // clamped for table addresses
-  const int kr = rc / nb;
+  const int kr = kpt(rc);
   const double sgn = (l & 1) ? -1.0 : 1.0;  // (-1)^x for x = 8 m + l
 
   F fft;
   fft.init(a.tw, l);
-  zd ur[R], acc[R], un[R];
+  zd ur[R], acc[R];
   static_for<0, R>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     ur[m] = rv ? gload(u + (size_t)NG * r + L * m + l) : mk(0.0, 0.0);
@@ -105,92 +116,95 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
   // zero the J sums
   for (int i = tid; i < 64 * kTilePitch; i += blockDim.x) acj[i] = mk(0.0, 0.0);
 
-  // pass list: off-diagonal 64 offsets, diagonal the 34 of diag_pass
-  auto partner = [&](int p1, int p0) {
-    return kTileRows * bj + 8 * ((wave + p1) & 7) + ((gw + p0) & 7);
+  // passes: off-diagonal all 64 offsets two-sided, diagonal those of kDiagRun
+  auto partner = [&](int q) {
+    return kTileRows * bj + 8 * ((wave + (q >> 3)) & 7) + ((gw + q) & 7);
   };
-  auto next_pass = [&](int q) {  // next pass index after q that runs, or 64
-    for (++q; q < 64; ++q) {
-      bool two;
-      if (!diag || diag_pass(q >> 3, q & 7, &two)) break;
-    }
-    return q;
-  };
-  auto fetch = [&](int s) {
-    const bool sv = s < rows;
-    const zd* urow = u + (size_t)NG * (sv ? s : 0);
+  // a partner past the last row is read from the last row; its table row is
+  // zeros (a.zrow), so it adds nothing to row r, and its own sum is never
+  // written out
+  auto fetch = [&](zd(&dst)[R], int s) {
+    const zd* urow = u + (size_t)NG * (s < rows ? s : rows - 1);
     static_for<0, R>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      un[m] = sv ? gload(urow + L * m + l) : mk(0.0, 0.0);
+      dst[m] = gload(urow + L * m + l);
     });
   };
-  int p = 0;  // pass 0 (offset (0,0)) runs in both kinds of tile
-  fetch(partner(0, 0));
-  lds_barrier();  // J sums zeroed
+  uint64_t left = diag ? kDiagRun : ~uint64_t(0);
+  const uint64_t two = diag ? kDiagBoth : ~uint64_t(0);
+  int win = 0;  // window (p1) of the pass in flight
 
-  for (int p1 = 0; p1 < 8; ++p1) {
-    for (; p < 64 && (p >> 3) == p1; p = next_pass(p)) {
-      const int p0 = p & 7;
-      bool both = true;
-      if (diag) diag_pass(p1, p0, &both);
-      const int s = partner(p1, p0);
-      const int sc = s < rows ? s : rows - 1;
-      const int ks = sc / nb;
-      zd us[R], v[R];
+  // One pass: the next offset of `left` with the partner in us; the partner
+  // of the pass after it is fetched into un (the last pass re-reads its own).
+  auto pass = [&](zd(&us)[R], zd(&un)[R]) {
+    const int p = __builtin_ctzll(left);
+    left &= left - 1;
+    // one barrier per window (empty windows included): in window p1 wave w
+    // owns the J rows of block (w + p1) % 8
+    for (; win < (p >> 3); ++win) lds_barrier();
+    const bool both = (two >> p) & 1;
+    const int s = partner(p);
+    const bool sv = s < rows;
+    const int sc = sv ? s : rows - 1;
+    const int ks = kpt(sc);
+    // the Coulomb row of (k_r, k_s) and the Nyquist difference D
+    const double* vt = (sv ? vtl + (size_t)(kr - ks + nk - 1) * NG : a.zrow) + l * R;
+    double vtr[R];
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      vtr[j] = gload(vt + j);
+    });
+    const double dnyq = gload(vtl + (size_t)(ks - kr + nk - 1) * NG + JNYQ) -
+                        gload(vtl + (size_t)(kr - ks + nk - 1) * NG + JNYQ);
+    fetch(un, left ? partner(__builtin_ctzll(left)) : s);
+    zd v[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[m] = zmulc(ur[m], us[m]);
+    });
+    fft.fwd(v, T, l);
+    const double nx = __shfl(v[JNYQ].x, gw * L, 64);
+    const double ny = __shfl(v[JNYQ].y, gw * L, 64);
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      v[j] = zscal(v[j], vtr[j]);
+    });
+    fft.inv(v, T, l);
+    // row r: c_rs u_s
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      acc[m].x = fma(v[m].x, us[m].x, acc[m].x);
+      acc[m].x = fma(-v[m].y, us[m].y, acc[m].x);
+      acc[m].y = fma(v[m].x, us[m].y, acc[m].y);
+      acc[m].y = fma(v[m].y, us[m].x, acc[m].y);
+    });
+    if (both) {
+      // row s: (conj(c_rs) + D conj(P(NG/2)) (-1)^x) u_r, into its LDS sum
+      const double fx = sgn * dnyq * nx, fy = -sgn * dnyq * ny;
+      zd* js = acj + (size_t)(s - kTileRows * bj) * kTilePitch;
       static_for<0, R>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        us[m] = un[m];
+        const double cx = v[m].x + fx, cy = -v[m].y + fy;
+        zd sm = js[L * m + l];
+        sm.x = fma(cx, ur[m].x, sm.x);
+        sm.x = fma(-cy, ur[m].y, sm.x);
+        sm.y = fma(cx, ur[m].y, sm.y);
+        sm.y = fma(cy, ur[m].x, sm.y);
+        js[L * m + l] = sm;
       });
-      // the Coulomb row of (k_r, k_s) and the Nyquist difference D
-      const double* vt = t.vtl + (size_t)(kr - ks + nk - 1) * NG + l * R;
-      double vtr[R];
-      static_for<0, R>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        vtr[j] = gload(vt + j);
-      });
-      const double dnyq = gload(t.vtl + (size_t)(ks - kr + nk - 1) * NG + JNYQ) -
-                          gload(t.vtl + (size_t)(kr - ks + nk - 1) * NG + JNYQ);
-      // prefetch the partner of the next pass (the last pass re-reads its own)
-      const int pn = next_pass(p);
-      fetch(pn < 64 ? partner(pn >> 3, pn & 7) : s);
-      static_for<0, R>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        v[m] = zmulc(ur[m], us[m]);
-      });
-      fft.fwd(v, T, l);
-      const double nx = __shfl(v[JNYQ].x, gw * L, 64);
-      const double ny = __shfl(v[JNYQ].y, gw * L, 64);
-      static_for<0, R>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        v[j] = zscal(v[j], vtr[j]);
-      });
-      fft.inv(v, T, l);
-      // row r: c_rs u_s
-      static_for<0, R>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        acc[m].x = fma(v[m].x, us[m].x, acc[m].x);
-        acc[m].x = fma(-v[m].y, us[m].y, acc[m].x);
-        acc[m].y = fma(v[m].x, us[m].y, acc[m].y);
-        acc[m].y = fma(v[m].y, us[m].x, acc[m].y);
-      });
-      if (both) {
-        // row s: (conj(c_rs) + D conj(P(NG/2)) (-1)^x) u_r, into its LDS sum
-        const double fx = sgn * dnyq * nx, fy = -sgn * dnyq * ny;
-        zd* js = acj + (size_t)(s - kTileRows * bj) * kTilePitch;
-        static_for<0, R>([&](auto mc) {
-          constexpr int m = decltype(mc)::value;
-          const double cx = v[m].x + fx, cy = -v[m].y + fy;
-          zd sm = js[L * m + l];
-          sm.x = fma(cx, ur[m].x, sm.x);
-          sm.x = fma(-cy, ur[m].y, sm.x);
-          sm.y = fma(cx, ur[m].y, sm.y);
-          sm.y = fma(cy, ur[m].x, sm.y);
-          js[L * m + l] = sm;
-        });
-      }
     }
-    lds_barrier();  // the next window hands the J blocks to other waves
+  };
+
+  // two partner buffers in turn (no copies between passes)
+  zd ua[R], ub[R];
+  fetch(ua, partner(0));  // pass 0 (offset (0,0)) runs in both kinds of tile
+  lds_barrier();          // J sums zeroed
+  while (left) {
+    pass(ua, ub);
+    if (!left) break;
+    pass(ub, ua);
   }
+  for (; win < 8; ++win) lds_barrier();  // the last windows; J sums complete
 
   // ---- per-row partials: slot c of row r holds its pairs with block c ----
   const int nblk = t.nblk;
