@@ -222,13 +222,13 @@ int device_cus() {
   return n;
 }
 
-template <int NG>
+template <int NG, bool XP = false>
 void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int ntraj, int nw, size_t lds,
                     hipStream_t s) {
   tdhf::ApplyArgs b = a;
   // wave-priority balancing pays only when the grid is one workgroup per CU
   b.balance = (long)rows * ntraj <= device_cus() ? 1 : 0;
-  hipLaunchKernelGGL(tdhf::apply_kernel<NG>, dim3(rows, ntraj), dim3(64 * nw), lds, s, b);
+  hipLaunchKernelGGL((tdhf::apply_kernel<NG, XP>), dim3(rows, ntraj), dim3(64 * nw), lds, s, b);
 }
 
 // rows workgroups per trajectory; ntraj > 1 for a uniform batch (grid y)

@@ -243,6 +243,9 @@ int batch_set_xmode(tdhf_batch* b, int xmode) {
     BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)tdhf::pairtile_lds_bytes()));
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::apply_kernel<64, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_for(64, 8, true)));
   }
   b->xmode = xmode;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -306,11 +309,15 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
     BHIP(b, hipGetLastError());
   }
   const size_t lds = lds_for(b->p.ng, nw, b->xmode != 0);
-  switch (b->p.ng) {
-    case 16: launch_apply_t<16>(a, b->ngrid, 1, nw, lds, b->stream); break;
-    case 32: launch_apply_t<32>(a, b->ngrid, 1, nw, lds, b->stream); break;
-    case 64: launch_apply_t<64>(a, b->ngrid, 1, nw, lds, b->stream); break;
-    case 128: launch_apply_t<128>(a, b->ngrid, 1, nw, lds, b->stream); break;
+  if (b->xmode) {  // pair tiles: NG = 64 (batch_pairtile_ok)
+    launch_apply_t<64, true>(a, b->ngrid, 1, nw, lds, b->stream);
+  } else {
+    switch (b->p.ng) {
+      case 16: launch_apply_t<16>(a, b->ngrid, 1, nw, lds, b->stream); break;
+      case 32: launch_apply_t<32>(a, b->ngrid, 1, nw, lds, b->stream); break;
+      case 64: launch_apply_t<64>(a, b->ngrid, 1, nw, lds, b->stream); break;
+      case 128: launch_apply_t<128>(a, b->ngrid, 1, nw, lds, b->stream); break;
+    }
   }
   BHIP(b, hipGetLastError());
   if (b->timing) BHIP(b, hipEventRecord(e1, b->stream));

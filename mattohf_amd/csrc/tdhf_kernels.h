@@ -224,7 +224,9 @@ __host__ __device__ constexpr size_t apply_lds_bytes(int nw, bool epilogue_only 
          apply_part_doubles(nw, NG) * sizeof(double);
 }
 
-template <int NG>
+// XP: the exchange and the density come from the pair tiles' partial sums
+// (TrajDesc.xpart; batched launches only): the pair loop is compiled out.
+template <int NG, bool XP = false>
 __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
   // exchange from the pair tiles: no transposes, only the epilogue scratch
-  const int ssz = xpart ? apply_epi_slot<NG>() : SSZ;
+  constexpr int ssz = XP ? apply_epi_slot<NG>() : SSZ;
   zd* T = smem + grp * ssz;
   zd* sW = smem + ngrp * ssz;
   zd* sVF = sW + NG;
@@ -334,7 +336,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // which go to LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   F fft;
-  if (!xpart) fft.init(a.tw, l);
+  if constexpr (!XP) fft.init(a.tw, l);
 
   zd acc[R], un[R];
   double den[R];
@@ -343,7 +345,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // u(q,j) of the next pair is fetched while the current pair is transformed;
   // the multipliers of the current pair are loaded before that prefetch, so
   // waiting for them leaves the prefetch in flight.
-  const int nitem = xpart ? 0 : nk * a.nbocc;  // pair tiles: no pairs here
+  const int nitem = XP ? 0 : nk * a.nbocc;  // pair tiles: no pairs here
   // item = jb + nbocc iq; a group's items are ngrp apart
   const int dq = ngrp / a.nbocc, dj = ngrp - dq * a.nbocc;
   int cq = grp / a.nbocc, cj = grp - cq * a.nbocc;  // item of the current pass
@@ -354,7 +356,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       un[m] = gload(urow + L * m + l);
     });
   };
-  if (grp < nitem) fetch(cq, cj);
+  if (!XP && grp < nitem) fetch(cq, cj);
   zd w_o = mk(0.0, 0.0), k1_o = mk(0.0, 0.0);
   double k2_o = 0.0;
   if (tid < NG) {
@@ -366,7 +368,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // (loaded with the other operands; used after the empty pair loop)
   zd xvf = mk(0.0, 0.0);
   double xrho = 0.0;
-  if (xpart && tid < NG) {
+  if (XP && tid < NG) {
     for (int c = 0; c < nblk; ++c) {
       xvf = zadd(xvf, gload(xpart + ((size_t)row * nblk + c) * NG + tid));
       xrho += gload(dpart + (size_t)c * NG + tid);
@@ -395,6 +397,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   lds_barrier();  // sW visible
 
   int it = 0;
+  if constexpr (!XP)
   for (int item = grp; item < nitem; item += ngrp, ++it) {
     if (a.balance) set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
     zd uq[R], v[R];
@@ -479,7 +482,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
-  if (xpart) {
+  if constexpr (XP) {
     if (tid < NG) {
       sVF[tid] = xvf;
       sRho[tid] = xrho;
