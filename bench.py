@@ -394,6 +394,89 @@ def adjoint_scale_bench(nt=8, seed=20261205, gs_iters=40):
     }
 
 
+def adjoint_full_bench(npulse=8, nt=8000, seed=20261206, gs_iters=40, fd=True):
+    """The batched adjoint at its full length: the a=12, NG=128, NK=128, NB=8
+    cell, npulse pulses of the many-pulse set (A0 log-uniform [0.01, 0.5],
+    omega uniform [0.03, 0.10] Ha, CEP uniform, sin^2 envelope T=400 au) over
+    all nt = 8000 steps (dt = 0.004 au), objective L = dt sum_n J_n^2.  The
+    forward keeps sqrt(nt)-spaced checkpoints under the default memory budget
+    (a quarter of the free device memory); the backward recomputes each
+    segment.  Reports the plain batched forward of the same pulses over the
+    same steps, the gradient cost relative to it, and central finite
+    differences of every pulse's L along one random direction (two more
+    checkpointed forwards).  Opt-in (--only adjoint-full): ~10 minutes at
+    npulse = 8; progress lines go to stderr."""
+    from mattohf_amd import tdhf as T
+
+    c = SCALE_CFG
+    ng, nk, nb, a, dt = c["ng"], c["nk"], c["nb"], c["acell"], c["dt"]
+
+    def say(msg):
+        print(f"adjoint-full: {msg}", file=sys.stderr, flush=True)
+
+    def progress(phase, done, total):
+        if done % 10 == 0 or done == total:
+            say(f"{'forward' if phase == 0 else 'backward'} segment {done}/{total}")
+
+    vion = T.ion_potential(ng, nk, a, [c["z"]], [c["alpha"]], [0.0])
+    G = T.TDHF(ng, nk, nb, c["nbocc"], a, dt, vion)
+    u = G.Gram_Schmidt(T.init_wf(ng, nb, nk, a))
+    ugs = G.GS_calculation(u, gs_iters, 4, tol=1e-11)[0]
+    A, Aave = batch_pulses(npulse, nt, dt, c["T"], seed)
+    us = [ugs] * npulse
+    B = T.Batch(ng, nb, c["nbocc"], a, dt, [nk] * npulse, np.array([vion] * npulse))
+    B.upload(us)
+    B.propagate(A[:, :2], Aave[:, :2])  # warm-up (the state is re-uploaded)
+    B.upload(us)
+    chunk, t_plain = 1000, 0.0
+    for i in range(0, nt, chunk):
+        t0 = time.perf_counter()
+        B.propagate(A[:, i:i + chunk], Aave[:, i:i + chunk])
+        t_plain += time.perf_counter() - t0
+        say(f"plain forward {min(nt, i + chunk)}/{nt} steps")
+    B.close()
+    G.set_progress(progress)
+    G.fwd_batch(us, A[:, :2], objective="jsq", want_obs=False)  # warm-up
+    t0 = time.perf_counter()
+    L, _, _, _, j = G.fwd_batch(us, A, objective="jsq")
+    t1 = time.perf_counter()
+    g, _ = G.bwd_batch(want_u0=False)
+    t2 = time.perf_counter()
+    full, seg = G.adjoint_info()
+    out = {
+        "workload": f"a=12, NG=128, NK=128, NB=8, {npulse} pulses per launch, objective "
+                    f"dt*sum J^2, all {nt} steps, dt={dt}",
+        "seed": seed,
+        "plain_batched_s": t_plain,
+        "plain_batched_traj_steps_per_s": npulse * nt / t_plain,
+        "plain_note": "tdhf_batch of the same pulses, E/N/J every step, in 1000-step calls",
+        "t_fwd_checkpointed_s": t1 - t0,
+        "t_bwd_s": t2 - t1,
+        "gradient_over_forward": (t2 - t0) / t_plain,
+        "tape": "full" if full else f"sqrt(N) checkpoints every {seg} steps",
+        "checkpoint_bytes": int(((nt + seg - 1) // seg + 4 * seg) * ng * nb * nk * 16 * npulse),
+        "L_range": [float(L.min()), float(L.max())],
+        "grad_finite": bool(np.isfinite(g).all()),
+    }
+    if fd:
+        rng = np.random.default_rng(seed + 1)
+        d = rng.standard_normal(A.shape)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        h = c["fd_h"]
+        say("finite differences: forward at A + h d")
+        lp = G.fwd_batch(us, A + h * d, objective="jsq", want_obs=False)[0]
+        say("finite differences: forward at A - h d")
+        lm = G.fwd_batch(us, A - h * d, objective="jsq", want_obs=False)[0]
+        fdv = (lp - lm) / (2 * h)
+        ad = np.einsum("tn,tn->t", g, d)
+        rel = np.abs(fdv - ad) / np.maximum(np.abs(fdv), 1e-300)
+        out.update({"fd_h": h, "fd_max_rel_err": float(rel.max()),
+                    "fd_rel_err": [float(x) for x in rel]})
+    G.set_progress(None)
+    G.close()
+    return out
+
+
 LEGS = ("batched", "adjoint", "adjoint-scale")
 
 
@@ -584,8 +667,10 @@ def main():
                          "baseline); for profiling runs")
     ap.add_argument("--adjoint-scale-steps", type=int, default=8)
     ap.add_argument("--leg-timeout", type=float, default=300.0)
-    ap.add_argument("--only", choices=list(LEGS), default=None,
-                    help="run one secondary leg alone and print its JSON")
+    ap.add_argument("--only", choices=list(LEGS) + ["adjoint-full"], default=None,
+                    help="run one secondary leg alone and print its JSON (adjoint-full: "
+                         "the 8000-step batched gradient, not part of the default run)")
+    ap.add_argument("--adjoint-full-pulses", type=int, default=8)
     ap.add_argument("--gs-iters", type=int, default=60,
                     help="ground-state iterations before the timed run (0: orthonormalised "
                          "initial guess; used for profiling runs)")
@@ -600,6 +685,10 @@ def main():
               "multi-GPU mode; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "adjoint-full":
+        print(json.dumps({"adjoint_full": adjoint_full_bench(npulse=args.adjoint_full_pulses)}),
+              flush=True)
+        return
     if args.only == "adjoint-scale":
         print(json.dumps({"adjoint_scale": adjoint_scale_bench(nt=args.adjoint_scale_steps)}),
               flush=True)

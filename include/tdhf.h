@@ -256,6 +256,16 @@ int tdhf_bwd_batch(tdhf_ctx* ctx, double* dLdA, double* dLdu0);
 /* How the last tdhf_fwd stored its states: full_tape = 1 if every stage state
  * is kept, else segment = checkpoint spacing S. */
 int tdhf_adjoint_info(tdhf_ctx* ctx, int* full_tape, int* segment);
+/* Progress of the checkpointed adjoint calls (tdhf_fwd_batch, tdhf_bwd_batch
+ * and their single-pulse forms): with fn set, the call waits for each
+ * checkpoint segment to finish on the device and then calls
+ * fn(user, phase, done, total) on the calling thread (phase 0: forward
+ * segments, 1: backward segments, each counted 1..total).  fn = NULL (the
+ * default) turns it off.  A full-tape run reports nothing.  The reference's
+ * counterpart is the RT wall time it prints every 1000 steps
+ * (src/RT_calculation.f90:56-58). */
+typedef void (*tdhf_progress_fn)(void* user, int phase, long done, long total);
+int tdhf_set_progress(tdhf_ctx* ctx, tdhf_progress_fn fn, void* user);
 
 /* ------------------------------------------------------------------------
  * Batched trajectories.  No counterpart in the reference (which runs one

@@ -54,6 +54,8 @@ struct tdhf_ctx {
   double acell = 0, dt = 0, dv = 0, vcell = 0, bx = 0, occ = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  tdhf_progress_fn progress = nullptr;  // checkpointed adjoint calls (tdhf_set_progress)
+  void* progress_user = nullptr;
   std::string err = "no error";
   std::vector<double> kx;
   // device tables

@@ -331,6 +331,21 @@ int adj_fail_free(AdjState& s, int rc) {
 
 }  // namespace
 
+// One progress report (tdhf_set_progress): after the segment has finished.
+int adj_report(tdhf_ctx* c, int phase, long done, long total) {
+  if (!c->progress) return 0;
+  CHKRET(sync(c));
+  c->progress(c->progress_user, phase, done, total);
+  return 0;
+}
+
+extern "C" int tdhf_set_progress(tdhf_ctx* c, tdhf_progress_fn fn, void* user) {
+  if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
+  c->progress = fn;
+  c->progress_user = user;
+  return 0;
+}
+
 extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const double* A,
                               int objective, const double* eref, double mem_budget, double* L,
                               double* etot, double* nele, double* jtot) {
@@ -393,10 +408,12 @@ extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const d
   if (s.full) {
     CHKRET(adj_forward_steps(c, s, 0, nt, s.tape));
   } else {
+    const long nseg = (nt + s.seg - 1) / s.seg;
     for (int s0 = 0; s0 < nt; s0 += s.seg) {
       HIPCHK(c, hipMemcpyAsync(s.ckpt + (size_t)(s0 / s.seg) * blk, s.st, bb,
                                hipMemcpyDeviceToDevice, c->stream));
       CHKRET(adj_forward_steps(c, s, s0, std::min(s.seg, nt - s0), nullptr));
+      CHKRET(adj_report(c, 0, s0 / s.seg + 1, nseg));
     }
   }
   // observables after the last step (slot nt) and the per-step reduction
@@ -490,12 +507,14 @@ extern "C" int tdhf_bwd_batch(tdhf_ctx* c, double* dLdA, double* dLdu0) {
   if (s.full) {
     CHKRET(adj_backward_steps(c, s, 0, nt, &ub, &ua, &ga, &gb));
   } else {
+    const long nseg = (nt + s.seg - 1) / s.seg;
     for (int s0 = ((nt - 1) / s.seg) * s.seg; s0 >= 0; s0 -= s.seg) {
       const int n = std::min(s.seg, nt - s0);
       HIPCHK(c, hipMemcpyAsync(s.st, s.ckpt + (size_t)(s0 / s.seg) * blk, blk * sizeof(zd),
                                hipMemcpyDeviceToDevice, c->stream));
       CHKRET(adj_forward_steps(c, s, s0, n, s.tape));
       CHKRET(adj_backward_steps(c, s, s0, n, &ub, &ua, &ga, &gb));
+      CHKRET(adj_report(c, 1, nseg - s0 / s.seg, nseg));
     }
     // s.st now holds the initial state: a second backward call would need the
     // final one, so the forward record is spent

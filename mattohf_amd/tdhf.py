@@ -33,6 +33,9 @@ class TDHFError(RuntimeError):
     pass
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.c_long, C.c_long)
+
+
 class Params(C.Structure):
     _fields_ = [("ng", C.c_int), ("nk", C.c_int), ("nb", C.c_int), ("nbocc", C.c_int),
                 ("acell", C.c_double), ("dt", C.c_double)]
@@ -65,6 +68,7 @@ SIGNATURES = {
     "tdhf_fwd": (_i, [_vp, _vp, _i, _vp, _d, _d, C.POINTER(_d), _vp, _vp, _vp]),
     "tdhf_bwd": (_i, [_vp, _vp, _vp]),
     "tdhf_adjoint_info": (_i, [_vp, C.POINTER(_i), C.POINTER(_i)]),
+    "tdhf_set_progress": (_i, [_vp, _vp, _vp]),
     "tdhf_fwd_batch": (_i, [_vp, _i, _vp, _i, _vp, _i, _vp, _d, _vp, _vp, _vp, _vp]),
     "tdhf_te_step": (_i, [_vp, _vp, _d, _i]),
     "tdhf_gs_run_diag": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _vp]),
@@ -436,6 +440,16 @@ class TDHF:
         _chk(load().tdhf_gs_run_diag(self.h, _ptr(u), int(nscf), int(ncg), _ptr(eh), _ptr(eps),
                                      _ptr(err), _ptr(dg)), self.h)
         return u, eh, eps, err, dg[:nscf]
+
+    def set_progress(self, fn):
+        """fn(phase, done, total), called after each checkpoint segment of the
+        checkpointed adjoint calls (phase 0 forward, 1 backward); None: off."""
+        if fn is None:
+            self._progress = None
+            _chk(load().tdhf_set_progress(self.h, None, None), self.h)
+            return
+        self._progress = PROGRESS_FN(lambda user, phase, done, total: fn(phase, done, total))
+        _chk(load().tdhf_set_progress(self.h, C.cast(self._progress, C.c_void_p), None), self.h)
 
     def adjoint_info(self):
         f, s = C.c_int(), C.c_int()

@@ -183,3 +183,28 @@ def test_batch_jsq_checkpointed_equals_full_tape():
     np.testing.assert_allclose(g2, g1, rtol=0, atol=1e-14 * np.abs(g1).max())
     for a_, b_ in zip(gu2, gu1):
         np.testing.assert_allclose(a_, b_, rtol=0, atol=1e-14 * np.abs(b_).max())
+
+
+def test_progress_reports_every_segment():
+    """tdhf_set_progress: one report per checkpoint segment, forward then
+    backward, counted 1..total; the gradient is the same with reports on."""
+    ng, nk, nb, a, z, nt, dt = 32, 4, 2, 8.0, 4.0, 29, 0.025
+    S, O, vion, u0, A0 = setup_case(ng, nk, nb, a, z, nt, dt)
+    As = np.stack([A0, 0.5 * A0])
+    state = 2 * ng * nb * nk * 16
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        G.fwd_batch([u0, u0], As, objective="jsq", mem_budget=30 * state)
+        g0, _ = G.bwd_batch(want_u0=False)
+        seen = []
+        G.set_progress(lambda phase, done, total: seen.append((phase, done, total)))
+        G.fwd_batch([u0, u0], As, objective="jsq", mem_budget=30 * state)
+        full, seg = G.adjoint_info()
+        g1, _ = G.bwd_batch(want_u0=False)
+        G.set_progress(None)
+        G.fwd_batch([u0, u0], As, objective="jsq", mem_budget=30 * state)
+        G.bwd_batch(want_u0=False)
+    nseg = (nt + seg - 1) // seg
+    assert not full and nseg > 1
+    assert seen == [(0, i, nseg) for i in range(1, nseg + 1)] + \
+        [(1, i, nseg) for i in range(1, nseg + 1)]
+    np.testing.assert_array_equal(g1, g0)
