@@ -164,9 +164,12 @@ def batch_pulses(ntraj, nt, dt, T, seed):
     return A, Aave
 
 
-def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
+def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True, exchange=None):
     """Batched run of len(nks) trajectories (NG, NB, a, Z of cfg; NK per
-    trajectory).  Each trajectory starts from the ground state of its NK."""
+    trajectory).  Each trajectory starts from the ground state of its NK.
+    exchange (0 row-wise, 1 pair tiles) fixes the exchange evaluation with one
+    wave per workgroup and the interleaved row schedule instead of the
+    autotuner's choice (for profiling runs)."""
     from mattohf_amd import tdhf as T
 
     ng, nb, a = cfg["ng"], cfg["nb"], cfg["acell"]
@@ -181,7 +184,11 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True):
     A, Aave = batch_pulses(ntraj, warmup + steps + 25, cfg["dt"], cfg["T"], seed)
     B = T.Batch(ng, nb, cfg["nbocc"], a, cfg["dt"], nks, np.array([vi[k] for k in nks]))
     B.upload([gs[k] for k in nks])
-    if autotune:
+    if exchange is not None:
+        B.set_waves(1)
+        B.set_schedule(1)
+        B.set_exchange(exchange)
+    elif autotune:
         B.autotune(A[:, :25], Aave[:, :25])
     if warmup:
         B.propagate(A[:, :warmup], Aave[:, :warmup])
@@ -671,6 +678,11 @@ def main():
                     help="run one secondary leg alone and print its JSON (adjoint-full: "
                          "the 8000-step batched gradient, not part of the default run)")
     ap.add_argument("--adjoint-full-pulses", type=int, default=8)
+    ap.add_argument("--batch-case", choices=["both", "mixed", "uniform"], default="both",
+                    help="batched leg: which batch to run (profiling runs)")
+    ap.add_argument("--batch-exchange", choices=["auto", "rows", "pairs"], default="auto",
+                    help="batched leg: the autotuner's exchange choice, or a fixed one "
+                         "(profiling runs)")
     ap.add_argument("--gs-iters", type=int, default=60,
                     help="ground-state iterations before the timed run (0: orthonormalised "
                          "initial guess; used for profiling runs)")
@@ -719,16 +731,20 @@ def batched_leg(args):
     cfg = dict(CFG)
     rng = np.random.default_rng(20261202)
     mixed = [int(k) for k in rng.choice([16, 32, 64], size=args.batch_traj)]
-    bm = batch_bench(cfg, mixed, args.batch_steps, 5, seed=20261202)
-    bu = batch_bench(cfg, [64] * max(1, args.batch_traj // 4), args.batch_steps, 5,
-                     seed=20261203)
-    return {
+    x = {"auto": None, "rows": 0, "pairs": 1}[args.batch_exchange]
+    out = {
         "workload": "independent trajectories, one launch per RK4 stage for all, "
-                    "hipGraph-replayed steps, autotuned waves/workgroup",
+                    "hipGraph-replayed steps, autotuned waves/workgroup, row schedule "
+                    "and exchange evaluation",
         "seed": 20261202,
-        "mixed_nk": bm,
-        "uniform_nk64": bu,
     }
+    if args.batch_case in ("both", "mixed"):
+        out["mixed_nk"] = batch_bench(cfg, mixed, args.batch_steps, 5, seed=20261202,
+                                      exchange=x)
+    if args.batch_case in ("both", "uniform"):
+        out["uniform_nk64"] = batch_bench(cfg, [64] * max(1, args.batch_traj // 4),
+                                          args.batch_steps, 5, seed=20261203, exchange=x)
+    return out
 
 
 if __name__ == "__main__":
