@@ -320,8 +320,8 @@ int tdhf_batch_get_schedule(const tdhf_batch* b);
  * ubkwbk_vFwbk, src/Functions.f90:247-257), 1 = pair tiles: since the stage
  * acts on its own state, the convolutions of (k,i),(q,j) and (q,j),(k,i) are
  * conjugates up to the Nyquist component, so one transform pair per
- * unordered pair feeds both rows (needs NG = 64 and NB == NBocc; TDHF_EINVAL
- * otherwise).  Results agree to rounding.  tdhf_batch_autotune includes
+ * unordered pair feeds both rows (needs NG = 64, NB == NBocc and fewer than
+ * 2^22 rows per trajectory; TDHF_EINVAL otherwise).  Results agree to rounding.  tdhf_batch_autotune includes
  * both. */
 int tdhf_batch_set_exchange(tdhf_batch* b, int mode);
 int tdhf_batch_get_exchange(const tdhf_batch* b);

@@ -173,7 +173,14 @@ int batch_set_schedule(tdhf_batch* b, int sched) {
   return 0;
 }
 
-bool batch_pairtile_ok(const tdhf_batch* b) { return b->p.ng == 64 && b->p.nb == b->p.nbocc; }
+// Pair tiles: NG = 64, every band occupied, and fewer than 2^22 rows per
+// trajectory (the kernel's float k-point division is exact below that).
+bool batch_pairtile_ok(const tdhf_batch* b) {
+  if (b->p.ng != 64 || b->p.nb != b->p.nbocc) return false;
+  for (int k : b->nk)
+    if ((long)k * b->p.nb >= (1L << 22)) return false;
+  return true;
+}
 
 // Pair-tile schedule: the tiles (bi <= bj) of every trajectory, trajectories
 // bin-packed over the 8 XCDs by their tiles' cost (an off-diagonal tile is 64
@@ -216,7 +223,7 @@ int batch_update_desc(tdhf_batch* b);
 // (1): partial buffers and the tile schedule are set up on first use.
 int batch_set_xmode(tdhf_batch* b, int xmode) {
   if (xmode && !batch_pairtile_ok(b))
-    return bfail(b, TDHF_EINVAL, "pair tiles need NG = 64 and NB == NBocc");
+    return bfail(b, TDHF_EINVAL, "pair tiles need NG = 64, NB == NBocc and NK*NB < 2^22");
   if (xmode && !b->d_xpart) {
     b->xoff.resize(b->ntraj);
     b->doff.resize(b->ntraj);
