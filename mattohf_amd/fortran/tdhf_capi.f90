@@ -206,6 +206,34 @@ module tdhf_capi
       real(c_double), intent(out) :: dLdA(*)
       complex(c_double_complex), intent(out) :: dLdu0(*)
     end function
+
+    ! how the last forward stored its states (full tape, or checkpoints every
+    ! segment steps)
+    integer(c_int) function tdhf_adjoint_info(ctx, full_tape, segment) &
+         bind(C, name="tdhf_adjoint_info")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(out) :: full_tape, segment
+    end function
+
+    ! progress callback of the checkpointed adjoint calls: fn is c_funloc of a
+    ! bind(C) subroutine with the interface tdhf_progress below, or
+    ! c_null_funptr to turn the reports off
+    integer(c_int) function tdhf_set_progress(ctx, fn, user) bind(C, name="tdhf_set_progress")
+      import :: c_int, c_ptr, c_funptr
+      type(c_ptr), value :: ctx
+      type(c_funptr), value :: fn
+      type(c_ptr), value :: user
+    end function
+  end interface
+
+  abstract interface
+    subroutine tdhf_progress(user, phase, done, total) bind(C)
+      import :: c_ptr, c_int, c_long
+      type(c_ptr), value :: user
+      integer(c_int), value :: phase
+      integer(c_long), value :: done, total
+    end subroutine
   end interface
 
 contains
