@@ -109,6 +109,9 @@ size_t adj_lds_for(int ng, int nw) {
 
 int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
   const int rows = c->nb * c->nk;
+  // the context's launch shape also for large uniform batches: at NG=128
+  // with 8 pulses, 2 or 4 waves per workgroup measured 15 % and 37 % slower
+  // than 8 (unlike the forward kernel, see launch_apply)
   int nw = choose_nw(c->ng, c->nk, c->nb);
   // fewer waves where the tiles would exceed the 160 KiB of LDS (NG = 128)
   while (nw > (c->ng + 63) / 64 && adj_lds_for(c->ng, nw) > 160 * 1024) --nw;
