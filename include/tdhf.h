@@ -311,8 +311,9 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
  * row, and so on which XCD): 0 = trajectories bin-packed over the XCDs (a
  * trajectory's rows on one XCD, contiguous; the default), 1 = the same with
  * the rows of an XCD's trajectories interleaved, 2 = no XCD pinning.  The
- * results do not depend on it.  tdhf_batch_autotune times every (schedule,
- * waves per workgroup) pair and keeps the fastest pair. */
+ * results do not depend on it.  tdhf_batch_autotune times every (exchange
+ * evaluation, schedule, waves per workgroup) combination and keeps the
+ * fastest. */
 int tdhf_batch_set_schedule(tdhf_batch* b, int sched);
 int tdhf_batch_get_schedule(const tdhf_batch* b);
 /* Exchange evaluation of the batched stages: 0 = row-wise (every ordered pair
@@ -321,8 +322,8 @@ int tdhf_batch_get_schedule(const tdhf_batch* b);
  * acts on its own state, the convolutions of (k,i),(q,j) and (q,j),(k,i) are
  * conjugates up to the Nyquist component, so one transform pair per
  * unordered pair feeds both rows (needs NG = 64, NB == NBocc and fewer than
- * 2^22 rows per trajectory; TDHF_EINVAL otherwise).  Results agree to rounding.  tdhf_batch_autotune includes
- * both. */
+ * 2^22 rows per trajectory; TDHF_EINVAL otherwise).  Results agree to
+ * rounding.  tdhf_batch_autotune includes both. */
 int tdhf_batch_set_exchange(tdhf_batch* b, int mode);
 int tdhf_batch_get_exchange(const tdhf_batch* b);
 int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);

@@ -227,6 +227,81 @@ module tdhf_capi
     end function
   end interface
 
+  ! batched trajectories (include/tdhf.h, "Batched trajectories"): nk(ntraj),
+  ! vion(NG,ntraj), u = the trajectories' ubk(NG,NB,nk(t)) back to back,
+  ! A(nt,ntraj), Aave(nt,ntraj), etot/nele/jtot(nt,ntraj), init3(3,ntraj)
+  interface
+    integer(c_int) function tdhf_batch_create(p, ntraj, nk, vion, b) bind(C, name="tdhf_batch_create")
+      import :: c_int, c_double, c_ptr, tdhf_params
+      type(tdhf_params), intent(in) :: p
+      integer(c_int), value :: ntraj
+      integer(c_int), intent(in) :: nk(*)
+      real(c_double), intent(in) :: vion(*)
+      type(c_ptr), intent(out) :: b
+    end function
+
+    integer(c_int) function tdhf_batch_destroy(b) bind(C, name="tdhf_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    type(c_ptr) function tdhf_batch_last_error_c(b) bind(C, name="tdhf_batch_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: b
+    end function
+
+    ! traj >= 0: one trajectory's orbitals; traj < 0: all of them
+    integer(c_int) function tdhf_batch_upload(b, traj, u) bind(C, name="tdhf_batch_upload")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: traj
+      complex(c_double_complex), intent(in) :: u(*)
+    end function
+
+    integer(c_int) function tdhf_batch_download(b, traj, u) bind(C, name="tdhf_batch_download")
+      import :: c_int, c_double_complex, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: traj
+      complex(c_double_complex), intent(out) :: u(*)
+    end function
+
+    integer(c_int) function tdhf_batch_propagate(b, nt, A, Aave, etot, nele, jtot, init3) &
+         bind(C, name="tdhf_batch_propagate")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nt
+      real(c_double), intent(in) :: A(*), Aave(*)
+      real(c_double), intent(out) :: etot(*), nele(*), jtot(*), init3(*)
+    end function
+
+    integer(c_int) function tdhf_batch_autotune(b, nt_trial, A, Aave, best_nw) &
+         bind(C, name="tdhf_batch_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nt_trial
+      real(c_double), intent(in) :: A(*), Aave(*)
+      integer(c_int), intent(out) :: best_nw
+    end function
+
+    integer(c_int) function tdhf_batch_set_waves(b, nw) bind(C, name="tdhf_batch_set_waves")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: nw
+    end function
+
+    integer(c_int) function tdhf_batch_set_schedule(b, sched) bind(C, name="tdhf_batch_set_schedule")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: sched
+    end function
+
+    integer(c_int) function tdhf_batch_set_exchange(b, mode) bind(C, name="tdhf_batch_set_exchange")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: mode
+    end function
+  end interface
+
   abstract interface
     subroutine tdhf_progress(user, phase, done, total) bind(C)
       import :: c_ptr, c_int, c_long
