@@ -191,12 +191,15 @@ def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
         np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
 
 
-def test_cfg3_pair_tiles_equal_single_runs(cfg3):
+@pytest.mark.parametrize("nw", [1, 2, 8])
+def test_cfg3_pair_tiles_equal_single_runs(cfg3, nw):
     """The exchange from pair tiles (one transform pair per unordered pair of
     rows, Nyquist correction for the reversed pair) against the row-wise
-    single runs, for NK = 16, 32, 64 (one, two, four 64-row blocks)."""
+    single runs, for NK = 16, 32, 64 (one, two, four 64-row blocks), with the
+    epilogue launch at 1, 2 and 8 waves per workgroup."""
     systems, us, vions, A, Aave, singles = cfg3
     with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+        B.set_waves(nw)
         B.set_exchange(1)
         assert B.exchange() == 1
         B.upload(us)
