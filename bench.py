@@ -484,6 +484,81 @@ def adjoint_full_bench(npulse=8, nt=8000, seed=20261206, gs_iters=40, fd=True):
     return out
 
 
+def batched_full_bench(nt=10000, ntraj=256, seed=20261202, gs_iters=60):
+    """The batched workload at its specified length: ntraj trajectories with NK
+    drawn from {16, 32, 64} (a=10, NG=64, NB=NBocc=4, Z=8), the seeded pulse
+    set of the batched leg, nt RK4 steps (dt = 0.01 au: 0.02 exceeds the RK4
+    stability bound of this grid), each from the GPU ground state of its NK,
+    autotuned launch.  The first trajectory of each NK is run again alone
+    (tdhf_rt_run on its own context) and its E(t), J(t) and final orbitals are
+    compared with the batch's over the whole run.  Opt-in (--only
+    batched-full); progress lines go to stderr."""
+    from mattohf_amd import tdhf as T
+
+    cfg = dict(CFG)
+    ng, nb, a, dt = cfg["ng"], cfg["nb"], cfg["acell"], cfg["dt"]
+
+    def say(msg):
+        print(f"batched-full: {msg}", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(seed)
+    nks = [int(k) for k in rng.choice([16, 32, 64], size=ntraj)]
+    gs, vi = {}, {}
+    for nk in sorted(set(nks)):
+        vion = T.ion_potential(ng, nk, a, [cfg["z"]], [cfg["alpha"]], [0.0])
+        with T.TDHF(ng, nk, nb, cfg["nbocc"], a, dt, vion) as G:
+            u = G.Gram_Schmidt(T.init_wf(ng, nb, nk, a))
+            gs[nk] = G.GS_calculation(u, gs_iters, 4, tol=1e-11)[0]
+        vi[nk] = vion
+    A, Aave = batch_pulses(ntraj, nt, dt, cfg["T"], seed)
+    B = T.Batch(ng, nb, cfg["nbocc"], a, dt, nks, np.array([vi[k] for k in nks]))
+    B.upload([gs[k] for k in nks])
+    B.autotune(A[:, :25], Aave[:, :25])
+    say(f"autotuned: {B.waves()} wave(s) per workgroup, row schedule {B.schedule()}, "
+        f"exchange {'pair tiles' if B.exchange() else 'row-wise'}")
+    chunk, wall, es, ns, js = 1000, 0.0, [], [], []
+    for i in range(0, nt, chunk):
+        t0 = time.perf_counter()
+        e, n, j, _ = B.propagate(A[:, i:i + chunk], Aave[:, i:i + chunk])
+        wall += time.perf_counter() - t0
+        es.append(e)
+        ns.append(n)
+        js.append(j)
+        say(f"{min(nt, i + chunk)}/{nt} steps")
+    finals = B.download()
+    waves, sched, xmode = B.waves(), B.schedule(), B.exchange()
+    B.close()
+    e, n, j = np.hstack(es), np.hstack(ns), np.hstack(js)
+    checks = {}
+    for nk in sorted(set(nks)):
+        t = nks.index(nk)
+        with T.TDHF(ng, nk, nb, cfg["nbocc"], a, dt, vi[nk]) as G:
+            u1, e1, n1, j1, _ = G.RT_calculation(gs[nk], A[t], Aave[t])
+        checks[str(nk)] = {
+            "trajectory": t,
+            "rel_dJ": float(np.abs(j[t] - j1).max() / np.abs(j1).max()),
+            "abs_dE_Ha": float(np.abs(e[t] - e1).max()),
+            "max_abs_du": float(np.abs(finals[t] - u1).max()),
+        }
+        say(f"NK={nk}: trajectory {t} checked against its single run")
+    return {
+        "workload": f"{ntraj} trajectories, NK from {{16, 32, 64}}, NG=64, NB=4, all {nt} "
+                    f"steps, dt={dt}, hipGraph-replayed stage launches",
+        "seed": seed,
+        "nk_counts": {str(k): int(sum(1 for x in nks if x == k)) for k in sorted(set(nks))},
+        "traj_steps_per_s": ntraj * nt / wall,
+        "wall_s": wall,
+        "waves_per_workgroup": waves,
+        "row_schedule": sched,
+        "exchange": "pair tiles" if xmode == 1 else "row-wise",
+        "observables_finite": bool(np.isfinite(e).all() and np.isfinite(j).all()),
+        "N_rel_drift_max": float(np.abs(n - n[:, :1]).max() / np.abs(n[:, :1]).min()),
+        "vs_single_runs": checks,
+        "vs_single_note": "one trajectory per NK through tdhf_rt_run on its own context "
+                          "(row-wise exchange), the whole run",
+    }
+
+
 LEGS = ("batched", "adjoint", "adjoint-scale")
 
 
@@ -674,9 +749,11 @@ def main():
                          "baseline); for profiling runs")
     ap.add_argument("--adjoint-scale-steps", type=int, default=8)
     ap.add_argument("--leg-timeout", type=float, default=300.0)
-    ap.add_argument("--only", choices=list(LEGS) + ["adjoint-full"], default=None,
+    ap.add_argument("--only", choices=list(LEGS) + ["adjoint-full", "batched-full"],
+                    default=None,
                     help="run one secondary leg alone and print its JSON (adjoint-full: "
-                         "the 8000-step batched gradient, not part of the default run)")
+                         "the 8000-step batched gradient; batched-full: the 10000-step "
+                         "mixed batch; neither is part of the default run)")
     ap.add_argument("--adjoint-full-pulses", type=int, default=8)
     ap.add_argument("--batch-case", choices=["both", "mixed", "uniform"], default="both",
                     help="batched leg: which batch to run (profiling runs)")
@@ -697,6 +774,9 @@ def main():
               "multi-GPU mode; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.only == "batched-full":
+        print(json.dumps({"batched_full": batched_full_bench()}), flush=True)
+        return
     if args.only == "adjoint-full":
         print(json.dumps({"adjoint_full": adjoint_full_bench(npulse=args.adjoint_full_pulses)}),
               flush=True)
