@@ -60,7 +60,13 @@ program mattohf_drv
   integer(c_int) :: rc, niter
   integer :: ik, ib, it, ig, iseed, ii
   real(8) :: rnd, rxc
+  ! wall times of the phases the reference reports (src/mATTOHF.f90:118-125)
+  real(8) :: ts, tpreps, tprepe, tGSs, tGSe, tRTs, tRTe, te
+  real(8), allocatable :: tchunk(:)
+  integer :: done, nstep
 
+  ts = wtime()
+  tpreps = ts
   write(*,*) 'You can specify a file for input data just below(If you do not have, just input [""]'
   read(*,*) input_data
   if (trim(input_data) == '') then
@@ -108,6 +114,8 @@ program mattohf_drv
   call tdhf_check(rc, ctx, 'tdhf_gram_schmidt')
 
   ! ground state
+  tprepe = wtime()
+  tGSs = tprepe
   allocate(etot_hist(0:Nscf), epsbk(NB,NK), epserr(NB,NK), diag(5,max(Nscf,1)))
   if (gs_tol > 0.d0) then
     rc = tdhf_gs_run(ctx, ubk, int(Nscf, c_int), int(NCG, c_int), gs_tol, etot_hist, epsbk, &
@@ -119,6 +127,8 @@ program mattohf_drv
     call tdhf_check(rc, ctx, 'tdhf_gs_run_diag')
     niter = Nscf
   end if
+  tGSe = wtime()
+  write(*,'("########SCF procedure is finished.########")')
   rc = tdhf_observables(ctx, ubk, 0.d0, etmp, ecomps, ntmp, jtmp)
   call tdhf_check(rc, ctx, 'tdhf_observables')
   open(10, file=trim(sys_name)//'_GSinfo.out', action='write')
@@ -181,7 +191,9 @@ program mattohf_drv
 
   ! real-time propagation
   allocate(time(NT), A(NT), Aave(NT), E(NT), Etot(NT), Nele(NT), Jtot(NT))
+  allocate(tchunk(NT/1000 + 1))
   call define_afield(time, A, Aave, E)
+  tRTs = wtime()
   select case (trim(TD_option))
   case ('TE', 'TEPC', 'TEPC2')
     ! the reference's Taylor propagators, one device step at a time
@@ -198,11 +210,36 @@ program mattohf_drv
       call tdhf_check(rc, ctx, 'tdhf_te_step')
       rc = tdhf_observables(ctx, ubk, A(it), Etot(it), ecomps, Nele(it), Jtot(it))
       call tdhf_check(rc, ctx, 'tdhf_observables')
+      if (mod(it, 1000) == 0) tchunk(it/1000) = wtime() - tRTs
     end do
   case default
-    rc = tdhf_rt_run(ctx, ubk, int(NT, c_int), A, Aave, Etot, Nele, Jtot, init3)
-    call tdhf_check(rc, ctx, 'tdhf_rt_run')
+    ! tdhf_rt_run in pieces of 1000 steps (each call returns when the device
+    ! has finished them), so that the reference's per-1000-step wall times can
+    ! be reported
+    call tdhf_check(tdhf_state_upload(ctx, ubk), ctx, 'tdhf_state_upload')
+    call tdhf_check(tdhf_rt_begin(ctx, int(NT, c_int), A, Aave), ctx, 'tdhf_rt_begin')
+    done = 0
+    do while (done < NT)
+      nstep = min(1000, NT - done)
+      call tdhf_check(tdhf_rt_advance(ctx, int(nstep, c_int)), ctx, 'tdhf_rt_advance')
+      done = done + nstep
+      if (mod(done, 1000) == 0) tchunk(done/1000) = wtime() - tRTs
+    end do
+    call tdhf_check(tdhf_rt_end(ctx, Etot, Nele, Jtot, init3), ctx, 'tdhf_rt_end')
+    call tdhf_check(tdhf_state_download(ctx, ubk), ctx, 'tdhf_state_download')
   end select
+  tRTe = wtime()
+  ! the reference's console lines of the time loop (src/RT_calculation.f90:37,54-60),
+  ! written after the run
+  write(*,'("#time(it), Etot(it)-Etot0, N(it)-N0, J(it), E(it), A(it)")')
+  do it = 1, NT
+    if (mod(it,100) == 0) write(*,'(i6,20e14.6)') it, time(it), Etot(it), Etot(it)-init3(1), &
+         Nele(it), Nele(it)-init3(2), Jtot(it), E(it), A(it)
+    if (mod(it,1000) == 0) &
+         write(*,'("## calc-time for ",i4," iterations :",e12.4," [sec] =",f12.8," [hour]##")') &
+         it, tchunk(it/1000), tchunk(it/1000)/3600.d0
+  end do
+  write(*,'("########Real-time propagation is finished.########")')
   open(10, file=trim(sys_name)//'_EtotNJEAt.out', action='write')
   write(10,'("#time(it), Etot(it)-Etot0, N(it)-N0, J(it), E(it), A(it)")')
   do it = 1, NT
@@ -238,8 +275,22 @@ program mattohf_drv
     call write_jeaf(trim(sys_name)//'_JEAF_window.out')
   end if
   rc = tdhf_destroy(ctx)
+  te = wtime()
+  write(*,'("The execution is finished successfully!")')
+  write(*,'("Time for preparation =",f12.8," [sec]")') tprepe-tpreps
+  write(*,'("Time for GS calculatoin =",e12.4," [sec] =",f12.8," [min]")') tGSe-tGSs, (tGSe-tGSs)/60.d0
+  write(*,'("Time for RT calculatoin =",e12.4," [sec] =",f12.8," [hour]")') tRTe-tRTs, (tRTe-tRTs)/3600.d0
+  write(*,'("Time for an interaction in RT =",e12.4," [sec]")') (tRTe-tRTs)/dble(NT)
+  write(*,'("Time for total  =",e12.4," [sec] =", f12.8," [hour]")') te-ts, (te-ts)/3600.d0
 
 contains
+
+  ! wall-clock seconds (the reference uses MPI_WTIME)
+  real(8) function wtime()
+    integer(8) :: c, r
+    call system_clock(c, r)
+    wtime = dble(c)/dble(r)
+  end function wtime
 
   subroutine write_jeaf(fname)
     character(*), intent(in) :: fname
