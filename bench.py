@@ -167,9 +167,9 @@ def batch_pulses(ntraj, nt, dt, T, seed):
 def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True, exchange=None):
     """Batched run of len(nks) trajectories (NG, NB, a, Z of cfg; NK per
     trajectory).  Each trajectory starts from the ground state of its NK.
-    exchange (0 row-wise, 1 pair tiles) fixes the exchange evaluation with one
-    wave per workgroup and the interleaved row schedule instead of the
-    autotuner's choice (for profiling runs)."""
+    exchange (0 row-wise; 1, 2, 3 pair tiles of 64, 32, 16 rows) fixes the
+    exchange evaluation, with one wave per workgroup and the interleaved row
+    schedule, instead of the autotuner's choice (for profiling runs)."""
     from mattohf_amd import tdhf as T
 
     ng, nb, a = cfg["ng"], cfg["nb"], cfg["acell"]
@@ -213,7 +213,7 @@ def batch_bench(cfg, nks, steps, warmup, seed, gs_iters=60, autotune=True, excha
         "ms_per_batch_step": wall / steps * 1e3,
         "waves_per_workgroup": waves,
         "row_schedule": sched,
-        "exchange": "pair tiles" if xmode == 1 else "row-wise",
+        "exchange": T.Batch.exchange_name(xmode),
         "avg_launch_us": avg_s * 1e6,
         "avg_launch_note": "HIP events around one RK4 stage of the whole batch (with pair "
                            "tiles: the pair-tile kernel and the fused stage kernel together)",
@@ -515,7 +515,7 @@ def batched_full_bench(nt=10000, ntraj=256, seed=20261202, gs_iters=60):
     B.upload([gs[k] for k in nks])
     B.autotune(A[:, :25], Aave[:, :25])
     say(f"autotuned: {B.waves()} wave(s) per workgroup, row schedule {B.schedule()}, "
-        f"exchange {'pair tiles' if B.exchange() else 'row-wise'}")
+        f"exchange {T.Batch.exchange_name(B.exchange())}")
     chunk, wall, es, ns, js = 1000, 0.0, [], [], []
     for i in range(0, nt, chunk):
         t0 = time.perf_counter()
@@ -550,7 +550,7 @@ def batched_full_bench(nt=10000, ntraj=256, seed=20261202, gs_iters=60):
         "wall_s": wall,
         "waves_per_workgroup": waves,
         "row_schedule": sched,
-        "exchange": "pair tiles" if xmode == 1 else "row-wise",
+        "exchange": T.Batch.exchange_name(xmode),
         "observables_finite": bool(np.isfinite(e).all() and np.isfinite(j).all()),
         "N_rel_drift_max": float(np.abs(n - n[:, :1]).max() / np.abs(n[:, :1]).min()),
         "vs_single_runs": checks,
@@ -757,7 +757,8 @@ def main():
     ap.add_argument("--adjoint-full-pulses", type=int, default=8)
     ap.add_argument("--batch-case", choices=["both", "mixed", "uniform"], default="both",
                     help="batched leg: which batch to run (profiling runs)")
-    ap.add_argument("--batch-exchange", choices=["auto", "rows", "pairs"], default="auto",
+    ap.add_argument("--batch-exchange", choices=["auto", "rows", "pairs", "pairs32", "pairs16"],
+                    default="auto",
                     help="batched leg: the autotuner's exchange choice, or a fixed one "
                          "(profiling runs)")
     ap.add_argument("--gs-iters", type=int, default=60,
@@ -811,7 +812,7 @@ def batched_leg(args):
     cfg = dict(CFG)
     rng = np.random.default_rng(20261202)
     mixed = [int(k) for k in rng.choice([16, 32, 64], size=args.batch_traj)]
-    x = {"auto": None, "rows": 0, "pairs": 1}[args.batch_exchange]
+    x = {"auto": None, "rows": 0, "pairs": 1, "pairs32": 2, "pairs16": 3}[args.batch_exchange]
     out = {
         "workload": "independent trajectories, one launch per RK4 stage for all, "
                     "hipGraph-replayed steps, autotuned waves/workgroup, row schedule "

@@ -318,12 +318,13 @@ int tdhf_batch_set_schedule(tdhf_batch* b, int sched);
 int tdhf_batch_get_schedule(const tdhf_batch* b);
 /* Exchange evaluation of the batched stages: 0 = row-wise (every ordered pair
  * of rows (k,i), (q,j) transformed by the fused kernel of row (k,i), as
- * ubkwbk_vFwbk, src/Functions.f90:247-257), 1 = pair tiles: since the stage
- * acts on its own state, the convolutions of (k,i),(q,j) and (q,j),(k,i) are
- * conjugates up to the Nyquist component, so one transform pair per
- * unordered pair feeds both rows (needs NG = 64, NB == NBocc and fewer than
- * 2^22 rows per trajectory; TDHF_EINVAL otherwise).  Results agree to
- * rounding.  tdhf_batch_autotune includes both. */
+ * ubkwbk_vFwbk, src/Functions.f90:247-257), 1, 2, 3 = pair tiles of 64, 32
+ * or 16 rows: since the stage acts on its own state, the convolutions of
+ * (k,i),(q,j) and (q,j),(k,i) are conjugates up to the Nyquist component, so
+ * one transform pair per unordered pair feeds both rows (needs NG = 64,
+ * NB == NBocc and fewer than 2^22 rows per trajectory; TDHF_EINVAL
+ * otherwise).  Smaller tiles make more workgroups and more partial sums per
+ * row.  Results agree to rounding.  tdhf_batch_autotune includes all four. */
 int tdhf_batch_set_exchange(tdhf_batch* b, int mode);
 int tdhf_batch_get_exchange(const tdhf_batch* b);
 int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);

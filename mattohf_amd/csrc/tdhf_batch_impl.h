@@ -49,12 +49,14 @@ struct tdhf_batch {
   int sched = 0;  // row schedule (build_schedule)
   // exchange: 0 = row-wise in the fused kernel, 1 = pair tiles
   // (pairtile_kernel, NG = 64 and NB == NBocc) + partial sums in the fused kernel
-  int xmode = 0;
+  int xmode = 0;      // 0 row-wise; 1, 2, 3: pair tiles of 64, 32, 16 rows
   int graph_xmode = 0;
+  int xrows = 0;      // tile rows the partial buffers and the tile list are laid out for
   zd* d_xpart = nullptr;
   double* d_dpart = nullptr;
   int4* d_tiles = nullptr;
   int ntiles = 0;
+  size_t tiles_cap = 0;  // entries d_tiles holds
   std::vector<size_t> xoff, doff;  // per trajectory (complex / double elements)
   size_t zoff = 0;                 // NG zeros in d_dpart after the partials
   bool timing = false;
@@ -156,6 +158,8 @@ void build_schedule(const tdhf_batch* b, std::vector<int2>& map, int sched) {
 }
 
 // Installs row schedule sched (device row map; the step graph is recaptured).
+int batch_build_tiles(tdhf_batch* b);
+
 int batch_set_schedule(tdhf_batch* b, int sched) {
   std::vector<int2> map;
   build_schedule(b, map, sched);
@@ -170,6 +174,8 @@ int batch_set_schedule(tdhf_batch* b, int sched) {
   b->sched = sched;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
+  // the pair-tile list follows the row schedule's XCD pinning
+  if (b->xmode) CHKRET(batch_build_tiles(b));
   return 0;
 }
 
@@ -182,28 +188,52 @@ bool batch_pairtile_ok(const tdhf_batch* b) {
   return true;
 }
 
-// Pair-tile schedule: the tiles (bi <= bj) of every trajectory, trajectories
-// bin-packed over the 8 XCDs by their tiles' cost (an off-diagonal tile is 64
-// passes, a diagonal one 34), each XCD's tiles costliest first, queues
-// interleaved so that entry b runs on XCD b % 8.
-void build_tile_schedule(const tdhf_batch* b, std::vector<int4>& map) {
+// Tile waves and rows of a pair-tile exchange mode (1, 2, 3: 8, 4, 2 waves).
+int xmode_waves(int xmode) { return 16 >> xmode; }
+int xmode_rows(int xmode) { return tdhf::tile_rows(xmode_waves(xmode)); }
+
+// Pair-tile schedule: the tiles (bi <= bj) of every trajectory, bin-packed
+// over the 8 XCDs by cost (an off-diagonal tile of W waves is 8 W passes, a
+// diagonal one 4 W + 2) -- with pin, whole trajectories (a trajectory's tiles
+// share one XCD's L2; row schedules 0 and 1), else tile by tile (row
+// schedule 2: small batches use the whole chip) -- each XCD's tiles costliest
+// first, queues interleaved so that entry b runs on XCD b % 8.
+void build_tile_schedule(const tdhf_batch* b, int xmode, bool pin, std::vector<int4>& map) {
   constexpr int kXcd = 8;
-  auto nblk = [&](int t) { return (b->p.nb * b->nk[t] + tdhf::kTileRows - 1) / tdhf::kTileRows; };
+  const int W = xmode_waves(xmode), TR = xmode_rows(xmode);
+  const double coff = 8.0 * W, cdiag = tdhf::diag_passes(W);
+  auto nblk = [&](int t) { return (b->p.nb * b->nk[t] + TR - 1) / TR; };
   auto cost = [&](int t) {
     const int n = nblk(t);
-    return 64.0 * n * (n - 1) / 2 + 34.0 * n;
+    return coff * n * (n - 1) / 2 + cdiag * n;
   };
   std::vector<int> order(b->ntraj);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
   std::vector<double> load(kXcd, 0.0);
   std::vector<std::vector<int4>> q(kXcd);
-  for (int t : order) {
-    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    load[x] += cost(t);
-    const int n = nblk(t);
-    for (int bi = 0; bi < n; ++bi)
-      for (int bj = bi; bj < n; ++bj) q[x].push_back(make_int4(t, bi, bj, 0));
+  auto lightest = [&] { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
+  if (pin) {
+    for (int t : order) {
+      const int x = lightest();
+      load[x] += cost(t);
+      const int n = nblk(t);
+      for (int bi = 0; bi < n; ++bi)
+        for (int bj = bi; bj < n; ++bj) q[x].push_back(make_int4(t, bi, bj, 0));
+    }
+  } else {
+    // off-diagonal tiles first (the costlier), each to the lightest XCD
+    for (int pass = 0; pass < 2; ++pass)
+      for (int t : order) {
+        const int n = nblk(t);
+        for (int bi = 0; bi < n; ++bi)
+          for (int bj = bi; bj < n; ++bj) {
+            if ((bi == bj) != (pass == 1)) continue;
+            const int x = lightest();
+            load[x] += bi == bj ? cdiag : coff;
+            q[x].push_back(make_int4(t, bi, bj, 0));
+          }
+      }
   }
   size_t longest = 0;
   for (auto& v : q) {
@@ -220,17 +250,27 @@ void build_tile_schedule(const tdhf_batch* b, std::vector<int4>& map) {
 int batch_update_desc(tdhf_batch* b);
 
 // Switches the exchange between the row-wise kernel (0) and the pair tiles
-// (1): partial buffers and the tile schedule are set up on first use.
+// of 64, 32 or 16 rows (1, 2, 3): partial buffers and the tile schedule are
+// set up when the tile size changes.
 int batch_set_xmode(tdhf_batch* b, int xmode) {
   if (xmode && !batch_pairtile_ok(b))
     return bfail(b, TDHF_EINVAL, "pair tiles need NG = 64, NB == NBocc and NK*NB < 2^22");
-  if (xmode && !b->d_xpart) {
+  if (xmode && b->xrows != xmode_rows(xmode)) {
+    const int TR = xmode_rows(xmode);
+    BHIP(b, hipStreamSynchronize(b->stream));  // no launch in flight uses the old buffers
+    dfree(b->d_xpart);
+    dfree(b->d_dpart);
+    dfree(b->d_tiles);
+    b->d_xpart = nullptr;
+    b->d_dpart = nullptr;
+    b->d_tiles = nullptr;
+    b->xrows = 0;
     b->xoff.resize(b->ntraj);
     b->doff.resize(b->ntraj);
     size_t xs = 0, ds = 0;
     for (int t = 0; t < b->ntraj; ++t) {
       const int rows = b->p.nb * b->nk[t];
-      const int nblk = (rows + tdhf::kTileRows - 1) / tdhf::kTileRows;
+      const int nblk = (rows + TR - 1) / TR;
       b->xoff[t] = xs;
       b->doff[t] = ds;
       xs += (size_t)rows * nblk * b->p.ng;
@@ -241,23 +281,43 @@ int batch_set_xmode(tdhf_batch* b, int xmode) {
     CHKRET(balloc(b, &b->d_dpart, ds + b->p.ng));
     b->zoff = ds;
     BHIP(b, hipMemsetAsync(b->d_dpart + ds, 0, b->p.ng * sizeof(double), b->stream));
-    std::vector<int4> map;
-    build_tile_schedule(b, map);
-    b->ntiles = (int)map.size();
-    CHKRET(balloc(b, &b->d_tiles, map.size()));
-    BHIP(b, hipMemcpyAsync(b->d_tiles, map.data(), map.size() * sizeof(int4),
-                           hipMemcpyHostToDevice, b->stream));
-    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel,
+    b->tiles_cap = 0;
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel<8>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)tdhf::pairtile_lds_bytes()));
+                                (int)tdhf::pairtile_lds_bytes(8)));
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel<4>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)tdhf::pairtile_lds_bytes(4)));
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel<2>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)tdhf::pairtile_lds_bytes(2)));
     BHIP(b, hipFuncSetAttribute((const void*)tdhf::apply_kernel<64, true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)lds_for(64, 8, true)));
+    b->xrows = TR;
   }
   b->xmode = xmode;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   b->gexec = nullptr;
+  if (xmode) CHKRET(batch_build_tiles(b));
   return batch_update_desc(b);
+}
+
+// (Re)builds the pair-tile list for the current tile size and row schedule.
+int batch_build_tiles(tdhf_batch* b) {
+  std::vector<int4> map;
+  build_tile_schedule(b, b->xmode, b->sched != 2, map);
+  if (map.size() > b->tiles_cap) {
+    BHIP(b, hipStreamSynchronize(b->stream));
+    dfree(b->d_tiles);
+    CHKRET(balloc(b, &b->d_tiles, map.size()));
+    b->tiles_cap = map.size();
+  }
+  BHIP(b, hipMemcpyAsync(b->d_tiles, map.data(), map.size() * sizeof(int4), hipMemcpyHostToDevice,
+                         b->stream));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->ntiles = (int)map.size();
+  return 0;
 }
 
 int batch_update_desc(tdhf_batch* b) {
@@ -279,7 +339,7 @@ int batch_update_desc(tdhf_batch* b) {
     d.Aave = b->d_Aave ? b->d_Aave + (size_t)b->A_cap * t : nullptr;
     d.obs = b->d_obs ? b->d_obs + b->obs_off[t] * (size_t)b->obs_cap : nullptr;
     const int rows = b->p.nb * b->nk[t];
-    d.nblk = (rows + tdhf::kTileRows - 1) / tdhf::kTileRows;
+    d.nblk = b->xmode ? (rows + b->xrows - 1) / b->xrows : 0;
     d.xpart = b->xmode ? b->d_xpart + b->xoff[t] : nullptr;
     d.dpart = b->xmode ? b->d_dpart + b->doff[t] : nullptr;
   }
@@ -311,8 +371,20 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
     pa.zrow = b->d_dpart + b->zoff;
     pa.nb = b->p.nb;
     pa.mode = a.mode;
-    hipLaunchKernelGGL(tdhf::pairtile_kernel, dim3(b->ntiles), dim3(512),
-                       tdhf::pairtile_lds_bytes(), b->stream, pa);
+    switch (xmode_waves(b->xmode)) {
+      case 8:
+        hipLaunchKernelGGL(tdhf::pairtile_kernel<8>, dim3(b->ntiles), dim3(512),
+                           tdhf::pairtile_lds_bytes(8), b->stream, pa);
+        break;
+      case 4:
+        hipLaunchKernelGGL(tdhf::pairtile_kernel<4>, dim3(b->ntiles), dim3(256),
+                           tdhf::pairtile_lds_bytes(4), b->stream, pa);
+        break;
+      default:
+        hipLaunchKernelGGL(tdhf::pairtile_kernel<2>, dim3(b->ntiles), dim3(128),
+                           tdhf::pairtile_lds_bytes(2), b->stream, pa);
+        break;
+    }
     BHIP(b, hipGetLastError());
   }
   const size_t lds = lds_for(b->p.ng, nw, b->xmode != 0);
@@ -542,7 +614,7 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
   if ((rc = batch_set_schedule(b, cached ? ct->second[1] : 0))) return bail(rc);
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
-  if (cached && ct->second[2] && (rc = batch_set_xmode(b, 1))) return bail(rc);
+  if (cached && ct->second[2] && (rc = batch_set_xmode(b, ct->second[2]))) return bail(rc);
   {
     const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
                      : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
@@ -662,7 +734,7 @@ int tdhf_batch_set_schedule(tdhf_batch* b, int sched) {
 int tdhf_batch_get_schedule(const tdhf_batch* b) { return b ? b->sched : -1; }
 
 int tdhf_batch_set_exchange(tdhf_batch* b, int mode) {
-  if (!b || mode < 0 || mode > 1) return bfail(b, TDHF_EINVAL, "invalid exchange mode %d", mode);
+  if (!b || mode < 0 || mode > 3) return bfail(b, TDHF_EINVAL, "invalid exchange mode %d", mode);
   return batch_set_xmode(b, mode);
 }
 
@@ -682,7 +754,7 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
   int best = b->nw, best_s = b->sched, best_x = b->xmode;
   double best_t = 1e300;
   const int cand[4] = {1, 2, 4, 8};
-  for (int xm = 0; xm <= (batch_pairtile_ok(b) ? 1 : 0); ++xm)
+  for (int xm = 0; xm <= (batch_pairtile_ok(b) ? 3 : 0); ++xm)
   for (int sc = 0; sc < kNumSchedules; ++sc) {
     CHKRET(batch_set_xmode(b, xm));
     CHKRET(batch_set_schedule(b, sc));

@@ -12,23 +12,25 @@
 //     D_kq = t_qk(NG/2) - t_kq(NG/2),  P = FFT[u_r conj(u_s)].
 // One transform pair therefore serves row r (c_rs u_s) and row s (c_sr u_r).
 //
-// pairtile_kernel computes, for one trajectory and one tile of 64 x 64 rows
-// (row blocks bi <= bj), every pair once and writes per-row partial sums;
-// the fused apply kernel then adds a row's partials (one per row block, in
-// block order) and continues with the Hartree and kinetic terms and the
-// stage update as for the row-wise exchange.
+// pairtile_kernel<W> computes, for one trajectory and one tile of TR x TR
+// rows (TR = 8 W; row blocks bi <= bj; W = 8, 4 or 2 waves), every pair once
+// and writes per-row partial sums; the fused apply kernel then adds a row's
+// partials (one per row block, in block order) and continues with the Hartree
+// and kinetic terms and the stage update as for the row-wise exchange.
+// Smaller tiles give more workgroups for small batches and more partials per
+// row; the batch autotuner chooses.
 //
 // Tile schedule (every ordered pair of a tile exactly once): group g = 8 w + gw
-// (wave w, group gw of the wave) owns row r = 64 bi + g and keeps its sum in
-// registers.  In pass (p1, p0) it pairs r with s = 64 bj + 8 ((w + p1) % 8)
+// (wave w, group gw of the wave) owns row r = TR bi + g and keeps its sum in
+// registers.  In pass (p1, p0) it pairs r with s = TR bj + 8 ((w + p1) % W)
 // + (gw + p0) % 8, whose sum lives in LDS.  For a fixed p1 (a window of
-// eight passes) wave w alone touches the LDS rows of block (w + p1) % 8, in
+// eight passes) wave w alone touches the LDS rows of block (w + p1) % W, in
 // program order, so the LDS sums need no atomics and their order is fixed;
 // a workgroup barrier separates the windows.  Off-diagonal tiles (bi < bj)
-// run all 64 offsets two-sided.  Diagonal tiles run the offsets o with
-// o < -o (mod 8 in each coordinate) two-sided and the four self-inverse ones
-// ((0,0): the self pairs; (0,4), (4,0), (4,4)) one-sided: 34 passes cover
-// all 64 x 64 ordered pairs.
+// run all 8 W offsets two-sided.  Diagonal tiles run the offsets o with
+// o < -o (mod W, mod 8) two-sided and the four self-inverse ones ((0,0): the
+// self pairs; (0,4), (W/2,0), (W/2,4)) one-sided: 4 W + 2 passes cover all
+// TR x TR ordered pairs (34 at W = 8).
 #pragma once
 #include "tdhf_kernels.h"
 
@@ -45,32 +47,44 @@ struct PairTileArgs {
   int mode;            // RK4 stage of the launch (selects the input state)
 };
 
-constexpr int kTileRows = 64;
 constexpr int kTilePitch = 72;  // complex elements per LDS row of the tile sums
 
-__host__ __device__ constexpr size_t pairtile_lds_bytes() {
-  // 64 transpose tiles (one per group) and 64 row sums, both 72 complex
-  return (size_t)2 * 64 * kTilePitch * sizeof(zd);
+// Tiles of W waves (W = 8, 4 or 2): 8 W rows per block (one per group), 8 W
+// passes per off-diagonal tile.
+__host__ __device__ constexpr int tile_rows(int W) { return 8 * W; }
+
+__host__ __device__ constexpr size_t pairtile_lds_bytes(int W) {
+  // one transpose tile per group and one row sum per row, both 72 complex
+  return (size_t)2 * tile_rows(W) * kTilePitch * sizeof(zd);
 }
 
-// The passes of a diagonal tile, as bit masks over p = 8 p1 + p0: the offsets
-// with o <= -o run, two-sided unless o == -o.
-__host__ __device__ constexpr uint64_t diag_pass_mask(bool two_sided) {
+// The passes of a diagonal tile of W waves, as bit masks over p = 8 p1 + p0
+// (p1 < W, p0 < 8): the offsets o = (p1, p0) with o <= -o (mod W, mod 8)
+// run, two-sided unless o == -o.
+__host__ __device__ constexpr uint64_t diag_pass_mask(int W, bool two_sided) {
   uint64_t m = 0;
-  for (int p = 0; p < 64; ++p) {
-    const int p1 = p >> 3, p0 = p & 7, m1 = (8 - p1) & 7, m0 = (8 - p0) & 7;
+  for (int p = 0; p < 8 * W; ++p) {
+    const int p1 = p >> 3, p0 = p & 7, m1 = (W - p1) & (W - 1), m0 = (8 - p0) & 7;
     const bool self = p1 == m1 && p0 == m0;
     const bool run = self || p1 < m1 || (p1 == m1 && p0 < m0);
     if (run && (!two_sided || !self)) m |= uint64_t(1) << p;
   }
   return m;
 }
-constexpr uint64_t kDiagRun = diag_pass_mask(false), kDiagBoth = diag_pass_mask(true);
-static_assert(__builtin_popcountll(kDiagRun) == 34 && __builtin_popcountll(kDiagBoth) == 30,
-              "diagonal tiles: 34 passes, 30 of them two-sided");
+// passes of a diagonal tile: 4 W + 2 (the four self-inverse offsets one-sided)
+__host__ __device__ constexpr int diag_passes(int W) { return 4 * W + 2; }
+static_assert(__builtin_popcountll(diag_pass_mask(8, false)) == diag_passes(8) &&
+                  __builtin_popcountll(diag_pass_mask(4, false)) == diag_passes(4) &&
+                  __builtin_popcountll(diag_pass_mask(2, false)) == diag_passes(2) &&
+                  __builtin_popcountll(diag_pass_mask(8, true)) == diag_passes(8) - 4,
+              "diagonal tiles: 4 W + 2 passes, four of them one-sided");
 
-__global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
-  constexpr int NG = 64;
+template <int W>
+__global__ void __launch_bounds__(64 * W) pairtile_kernel(PairTileArgs a) {
+  static_assert(W == 2 || W == 4 || W == 8, "tiles of 2, 4 or 8 waves");
+  constexpr int NG = 64, TR = tile_rows(W);
+  constexpr uint64_t kDiagRun = diag_pass_mask(W, false), kDiagBoth = diag_pass_mask(W, true);
+  constexpr uint64_t kAll = W == 8 ? ~uint64_t(0) : (uint64_t(1) << (8 * W)) - 1;
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L;  // 8 x 8
   constexpr int JNYQ = F::NYQ_REG;   // lane 0 of a group holds G index NG/2 here
@@ -98,8 +112,8 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
   const int l = lane % L, gw = lane / L;
   const int g = wave * 8 + gw;
   zd* T = smem + g * kTilePitch;           // this group's transpose tile
-  zd* acj = smem + 64 * kTilePitch;        // row sums of the J block
-  const int r = kTileRows * bi + g;
+  zd* acj = smem + TR * kTilePitch;        // row sums of the J block
+  const int r = TR * bi + g;
   const bool rv = r < rows;
   const int rc = rv ? r : rows - 1;        // clamped for table addresses
   const int kr = kpt(rc);
@@ -114,11 +128,11 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
     acc[m] = mk(0.0, 0.0);
   });
   // zero the J sums
-  for (int i = tid; i < 64 * kTilePitch; i += blockDim.x) acj[i] = mk(0.0, 0.0);
+  for (int i = tid; i < TR * kTilePitch; i += blockDim.x) acj[i] = mk(0.0, 0.0);
 
-  // passes: off-diagonal all 64 offsets two-sided, diagonal those of kDiagRun
+  // passes: off-diagonal all 8 W offsets two-sided, diagonal those of kDiagRun
   auto partner = [&](int q) {
-    return kTileRows * bj + 8 * ((wave + (q >> 3)) & 7) + ((gw + q) & 7);
+    return TR * bj + 8 * ((wave + (q >> 3)) & (W - 1)) + ((gw + q) & 7);
   };
   // a partner past the last row is read from the last row; its table row is
   // zeros (a.zrow), so it adds nothing to row r, and its own sum is never
@@ -130,8 +144,8 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
       dst[m] = gload(urow + L * m + l);
     });
   };
-  uint64_t left = diag ? kDiagRun : ~uint64_t(0);
-  const uint64_t two = diag ? kDiagBoth : ~uint64_t(0);
+  uint64_t left = diag ? kDiagRun : kAll;
+  const uint64_t two = diag ? kDiagBoth : kAll;
   int win = 0;  // window (p1) of the pass in flight
 
   // One pass: the next offset of `left` with the partner in us; the partner
@@ -140,7 +154,7 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
     const int p = __builtin_ctzll(left);
     left &= left - 1;
     // one barrier per window (empty windows included): in window p1 wave w
-    // owns the J rows of block (w + p1) % 8
+    // owns the J rows of block (w + p1) % W
     for (; win < (p >> 3); ++win) lds_barrier();
     const bool both = (two >> p) & 1;
     const int s = partner(p);
@@ -181,7 +195,7 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
     if (both) {
       // row s: (conj(c_rs) + D conj(P(NG/2)) (-1)^x) u_r, into its LDS sum
       const double fx = sgn * dnyq * nx, fy = -sgn * dnyq * ny;
-      zd* js = acj + (size_t)(s - kTileRows * bj) * kTilePitch;
+      zd* js = acj + (size_t)(s - TR * bj) * kTilePitch;
       static_for<0, R>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const double cx = v[m].x + fx, cy = -v[m].y + fy;
@@ -204,7 +218,7 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
     if (!left) break;
     pass(ub, ua);
   }
-  for (; win < 8; ++win) lds_barrier();  // the last windows; J sums complete
+  for (; win < W; ++win) lds_barrier();  // the last windows; J sums complete
 
   // ---- per-row partials: slot c of row r holds its pairs with block c ----
   const int nblk = t.nblk;
@@ -219,9 +233,9 @@ __global__ void __launch_bounds__(512) pairtile_kernel(PairTileArgs a) {
     });
   }
   if (!diag) {
-    for (int i = tid; i < 64 * NG; i += blockDim.x) {
+    for (int i = tid; i < TR * NG; i += blockDim.x) {
       const int j = i / NG, x = i - j * NG;
-      const int s = kTileRows * bj + j;
+      const int s = TR * bj + j;
       if (s < rows) gstore(t.xpart + ((size_t)s * nblk + bi) * NG + x, acj[(size_t)j * kTilePitch + x]);
     }
   } else {

@@ -557,11 +557,16 @@ class Batch:
         return load().tdhf_batch_get_schedule(self.h)
 
     def set_exchange(self, mode):
-        """0: row-wise exchange; 1: pair tiles (NG=64, NB == NBocc)."""
+        """0: row-wise exchange; 1, 2, 3: pair tiles of 64, 32, 16 rows (NG=64,
+        NB == NBocc)."""
         _bchk(load().tdhf_batch_set_exchange(self.h, int(mode)), self.h)
 
     def exchange(self):
         return load().tdhf_batch_get_exchange(self.h)
+
+    @staticmethod
+    def exchange_name(mode):
+        return "row-wise" if mode == 0 else f"pair tiles of {128 >> mode} rows"
 
     def autotune(self, A, Aave):
         A = np.ascontiguousarray(A, dtype=np.float64)

@@ -191,17 +191,20 @@ def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
         np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("nw", [1, 2, 8])
-def test_cfg3_pair_tiles_equal_single_runs(cfg3, nw):
+@pytest.mark.parametrize("mode,nw,sched", [(1, 1, 0), (1, 2, 1), (1, 8, 2), (2, 1, 2),
+                                           (3, 1, 0), (3, 4, 2)])
+def test_cfg3_pair_tiles_equal_single_runs(cfg3, mode, nw, sched):
     """The exchange from pair tiles (one transform pair per unordered pair of
     rows, Nyquist correction for the reversed pair) against the row-wise
-    single runs, for NK = 16, 32, 64 (one, two, four 64-row blocks), with the
-    epilogue launch at 1, 2 and 8 waves per workgroup."""
+    single runs, for NK = 16, 32, 64, with tiles of 64, 32 and 16 rows (modes
+    1, 2, 3), the epilogue launch at 1 to 8 waves per workgroup, and tile
+    lists pinned to XCDs per trajectory (row schedules 0, 1) or not (2)."""
     systems, us, vions, A, Aave, singles = cfg3
     with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
         B.set_waves(nw)
-        B.set_exchange(1)
-        assert B.exchange() == 1
+        B.set_exchange(mode)
+        B.set_schedule(sched)
+        assert B.exchange() == mode and B.schedule() == sched
         B.upload(us)
         e, n, j, i3 = B.propagate(A, Aave)
         finals = B.download()
@@ -214,9 +217,11 @@ def test_cfg3_pair_tiles_equal_single_runs(cfg3, nw):
         np.testing.assert_allclose(finals[t], u1, rtol=0, atol=1e-12)
 
 
-def test_pair_tiles_partial_blocks_match_oracle():
-    """Row counts that are not multiples of the 64-row block (NK = 10: 40
-    rows; NK = 20: 80 rows; NK = 3: 12 rows), against the oracle."""
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_pair_tiles_partial_blocks_match_oracle(mode):
+    """Row counts that are not multiples of the row block (NK = 10: 40 rows;
+    NK = 20: 80 rows; NK = 3: 12 rows), against the oracle, for each tile
+    size; the same batch object also switches between tile sizes."""
     ng, nb, a, dt, nt, nks = 64, 4, 10.0, 0.01, 8, [10, 20, 3]
     us, vions, systems = [], [], []
     for t, nk in enumerate(nks):
@@ -227,7 +232,10 @@ def test_pair_tiles_partial_blocks_match_oracle():
     A, Aave = seeded_pulses(len(nks), 20000 + nt, dt, 400.0, 20261206)
     A, Aave = np.ascontiguousarray(A[:, 20000:]), np.ascontiguousarray(Aave[:, 20000:])
     with T.Batch(ng, nb, nb, a, dt, nks, np.array(vions)) as B:
-        B.set_exchange(1)
+        B.set_exchange(1 + mode % 3)  # another tile size first: buffers are rebuilt
+        B.upload(us)
+        B.propagate(A[:, :2], Aave[:, :2])
+        B.set_exchange(mode)
         B.upload(us)
         e, n, j, _ = B.propagate(A, Aave)
         finals = B.download()
