@@ -7,8 +7,11 @@
 // schedule (rowmap) lists rows longest-first within each XCD, and pins each
 // trajectory to one XCD (workgroups are dispatched round-robin over the 8
 // XCDs, so entry b runs on XCD b % 8) so that its orbitals stay in that XCD's
-// L2.  The RK4 steps are captured as a hipGraph and replayed; the launch
-// shape (waves per workgroup) is picked by tdhf_batch_autotune.
+// L2.  The exchange is evaluated row-wise by that kernel or, for NG = 64 with
+// every band occupied, by pair tiles (tdhf_pairtile.h) whose per-row partial
+// sums the kernel then adds.  The RK4 steps are captured as a hipGraph and
+// replayed; the exchange evaluation and tile size, the row schedule and the
+// launch shape (waves per workgroup) are picked by tdhf_batch_autotune.
 #pragma once
 
 #include <algorithm>
@@ -47,9 +50,9 @@ struct tdhf_batch {
   int graph_sched = 0;
   int nw = 4;
   int sched = 0;  // row schedule (build_schedule)
-  // exchange: 0 = row-wise in the fused kernel, 1 = pair tiles
-  // (pairtile_kernel, NG = 64 and NB == NBocc) + partial sums in the fused kernel
-  int xmode = 0;      // 0 row-wise; 1, 2, 3: pair tiles of 64, 32, 16 rows
+  // exchange: 0 = row-wise in the fused kernel; 1, 2, 3 = pair tiles of 64,
+  // 32, 16 rows (pairtile_kernel<8, 4, 2>) + partial sums in the fused kernel
+  int xmode = 0;
   int graph_xmode = 0;
   int xrows = 0;      // tile rows the partial buffers and the tile list are laid out for
   zd* d_xpart = nullptr;
@@ -157,9 +160,10 @@ void build_schedule(const tdhf_batch* b, std::vector<int2>& map, int sched) {
   while (!map.empty() && map.back().x < 0) map.pop_back();
 }
 
-// Installs row schedule sched (device row map; the step graph is recaptured).
 int batch_build_tiles(tdhf_batch* b);
 
+// Installs row schedule sched (device row map and, with pair tiles, the tile
+// list; the step graph is recaptured).
 int batch_set_schedule(tdhf_batch* b, int sched) {
   std::vector<int2> map;
   build_schedule(b, map, sched);
