@@ -46,9 +46,26 @@ struct AdjState {
 
 namespace {
 
+// The forward record of each context.  Contexts may be used from different
+// host threads (one thread per context), so the map itself is guarded; a
+// context's record is only touched by its own thread (references into a
+// std::map stay valid while other keys are added or removed).
 std::map<const tdhf_ctx*, AdjState>& adj_states() {
   static std::map<const tdhf_ctx*, AdjState> m;
   return m;
+}
+std::mutex& adj_mutex() {
+  static std::mutex m;
+  return m;
+}
+AdjState& adj_state_of(const tdhf_ctx* c) {
+  std::lock_guard<std::mutex> lk(adj_mutex());
+  return adj_states()[c];
+}
+AdjState* adj_find(const tdhf_ctx* c) {
+  std::lock_guard<std::mutex> lk(adj_mutex());
+  auto it = adj_states().find(c);
+  return it == adj_states().end() ? nullptr : &it->second;
 }
 
 void adj_free(AdjState& s) {
@@ -358,7 +375,7 @@ extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const d
     return fail(c, TDHF_EINVAL, "unknown objective %d", objective);
   if (c->nb != c->nbocc)
     return fail(c, TDHF_EINVAL, "the adjoint path propagates occupied bands only (NB must equal NBocc)");
-  AdjState& s = adj_states()[c];
+  AdjState& s = adj_state_of(c);
   adj_free(s);
   s.nt = nt;
   s.ntraj = ntraj;
@@ -481,10 +498,10 @@ extern "C" int tdhf_fwd_batch(tdhf_ctx* c, int ntraj, double* u, int nt, const d
 
 extern "C" int tdhf_bwd_batch(tdhf_ctx* c, double* dLdA, double* dLdu0) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
-  auto itS = adj_states().find(c);
-  if (itS == adj_states().end() || !itS->second.have_fwd)
+  AdjState* ps = adj_find(c);
+  if (!ps || !ps->have_fwd)
     return fail(c, TDHF_EINVAL, "the backward call needs a preceding forward call on this context");
-  AdjState& s = itS->second;
+  AdjState& s = *ps;
   if (!dLdA) return fail(c, TDHF_EINVAL, "dLdA is NULL");
   if (!s.full && !s.ckpt)
     return fail(c, TDHF_EINVAL, "the checkpointed forward state was consumed by an earlier backward call");
@@ -562,20 +579,19 @@ extern "C" int tdhf_bwd(tdhf_ctx* c, double* dLdA, double* dLdu0) {
 
 extern "C" int tdhf_adjoint_info(tdhf_ctx* c, int* full_tape, int* segment) {
   if (!c) return fail(nullptr, TDHF_EINVAL, "ctx is NULL");
-  auto it = adj_states().find(c);
-  if (it == adj_states().end() || !it->second.have_fwd)
-    return fail(c, TDHF_EINVAL, "no forward run recorded");
-  if (full_tape) *full_tape = it->second.full ? 1 : 0;
-  if (segment) *segment = it->second.seg;
+  const AdjState* ps = adj_find(c);
+  if (!ps || !ps->have_fwd) return fail(c, TDHF_EINVAL, "no forward run recorded");
+  if (full_tape) *full_tape = ps->full ? 1 : 0;
+  if (segment) *segment = ps->seg;
   return 0;
 }
 
 namespace {
 void adj_release(const tdhf_ctx* c) {
-  auto it = adj_states().find(c);
-  if (it != adj_states().end()) {
-    adj_free(it->second);
-    adj_states().erase(it);
-  }
+  AdjState* ps = adj_find(c);
+  if (!ps) return;
+  adj_free(*ps);
+  std::lock_guard<std::mutex> lk(adj_mutex());
+  adj_states().erase(c);
 }
 }  // namespace

@@ -527,10 +527,27 @@ void batch_release(tdhf_batch* b) {
   b->stream = nullptr;
 }
 
-// autotuner cache: shape -> (waves per workgroup, row schedule, exchange mode)
+// autotuner cache: shape -> (waves per workgroup, row schedule, exchange
+// mode), shared by the batches of the process (guarded: batches may be
+// created and tuned on different host threads)
 std::map<std::vector<int>, std::array<int, 3>>& tune_cache() {
   static std::map<std::vector<int>, std::array<int, 3>> m;
   return m;
+}
+std::mutex& tune_mutex() {
+  static std::mutex m;
+  return m;
+}
+bool tune_lookup(const std::vector<int>& key, std::array<int, 3>* out) {
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  auto it = tune_cache().find(key);
+  if (it == tune_cache().end()) return false;
+  *out = it->second;
+  return true;
+}
+void tune_store(const std::vector<int>& key, const std::array<int, 3>& v) {
+  std::lock_guard<std::mutex> lk(tune_mutex());
+  tune_cache()[key] = v;
 }
 
 std::vector<int> tune_key(const tdhf_batch* b) {
@@ -611,14 +628,14 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
   // fewest waves per workgroup that cover the grid (one at NG <= 64: the
   // groups of a single wave need no workgroup barrier, and eight such
   // workgroups share a CU) and the XCD-pinned schedule
-  auto ct = tune_cache().find(tune_key(b));
-  const bool cached = ct != tune_cache().end();
-  b->nw = cached ? ct->second[0] : (p->ng + 63) / 64;
+  std::array<int, 3> ct{};
+  const bool cached = tune_lookup(tune_key(b), &ct);
+  b->nw = cached ? ct[0] : (p->ng + 63) / 64;
   b->ngrid = 0;
-  if ((rc = batch_set_schedule(b, cached ? ct->second[1] : 0))) return bail(rc);
+  if ((rc = batch_set_schedule(b, cached ? ct[1] : 0))) return bail(rc);
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
-  if (cached && ct->second[2] && (rc = batch_set_xmode(b, ct->second[2]))) return bail(rc);
+  if (cached && ct[2] && (rc = batch_set_xmode(b, ct[2]))) return bail(rc);
   {
     const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
                      : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
@@ -792,7 +809,7 @@ int tdhf_batch_autotune(tdhf_batch* b, int nt_trial, const double* A, const doub
   b->nw = best;
   CHKRET(batch_set_xmode(b, best_x));
   CHKRET(batch_set_schedule(b, best_s));
-  tune_cache()[tune_key(b)] = {best, best_s, best_x};
+  tune_store(tune_key(b), {best, best_s, best_x});
   if (best_nw) *best_nw = best;
   return 0;
 }
