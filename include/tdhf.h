@@ -16,8 +16,9 @@
  *    streams and graphs.  Calls are synchronous at return.
  *  - Return value 0 = success, negative = error (TDHF_E*); the message is
  *    available from tdhf_last_error().  No exception crosses the ABI.
- *  - A context is used by one host thread at a time; several contexts may
- *    coexist (each owns its own device buffers and stream).
+ *  - A context (or batch) is used by one host thread at a time; several may
+ *    coexist and be used from different threads at once (each owns its own
+ *    device buffers and stream; the library's shared records are guarded).
  *  - Everything is FP64 / complex128.
  */
 #ifndef TDHF_H
