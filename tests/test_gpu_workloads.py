@@ -1,13 +1,14 @@
 """GPU parity at the benchmark workload shapes, through the C-ABI, against the
 CPU oracle (and, for the adjoint, against central finite differences).
 
-  cfg2  plain       a=10, NG=64,  NK=64,  NB=NBocc=4, Z=8  (8-wave launches)
-  cfg3  batched     NG=64, NB=4, NK in {16, 32, 64} in one batch, seeded
-                    pulses (A0 log-uniform [0.01, 0.5], w uniform
-                    [0.03, 0.10] Ha, CEP uniform), every launch shape
-  cfg4  adjoint     a=8, NG=32, NK=32, NB=2, Z=4, 4000 steps dt=0.025, pulse
-                    = 200 piecewise-linear knots ~ N(0, 0.05^2) x sin^2
-  cfg5  at scale    a=12, NG=128, NK=128, NB=NBocc=8, Z=16, objective dt sum J^2
+  ng64_nk64     plain      a=10, NG=64, NK=64, NB=NBocc=4, Z=8 (8-wave launches)
+  mixed_batch   batched    NG=64, NB=4, NK in {16, 32, 64} in one batch, seeded
+                           pulses (A0 log-uniform [0.01, 0.5], w uniform
+                           [0.03, 0.10] Ha, CEP uniform), every launch shape
+  adjoint_ng32  adjoint    a=8, NG=32, NK=32, NB=2, Z=4, 4000 steps dt=0.025,
+                           pulse = 200 piecewise-linear knots ~ N(0, 0.05^2) x sin^2
+  ng128_nk128   at scale   a=12, NG=128, NK=128, NB=NBocc=8, Z=16, objective
+                           dt sum J^2
 
 Tolerances: operators max|d| / max|ref| < 1e-12; trajectories |dE| < 1e-10 Ha
 and max_t|dJ| / max_t|J| < 1e-8; batched trajectory = its single run to
@@ -44,7 +45,7 @@ def random_orbitals(S, seed):
 
 
 def seeded_pulses(ntraj, nt, dt, T_, seed):
-    """cfg3 pulse set: A0 log-uniform [0.01, 0.5], w uniform [0.03, 0.10] Ha,
+    """mixed_batch pulse set: A0 log-uniform [0.01, 0.5], w uniform [0.03, 0.10] Ha,
     CEP uniform [0, 2 pi], A(t) = A0 sin^2(pi t/T) sin(w t + CEP)."""
     rng = np.random.default_rng(seed)
     A0 = np.exp(rng.uniform(math.log(0.01), math.log(0.5), ntraj))
@@ -58,13 +59,13 @@ def seeded_pulses(ntraj, nt, dt, T_, seed):
 
 
 # --------------------------------------------------------------------------
-# cfg2: a=10, NG=64, NK=64, NB=NBocc=4, Z=8
+# ng64_nk64: a=10, NG=64, NK=64, NB=NBocc=4, Z=8
 # --------------------------------------------------------------------------
 C2 = dict(ng=64, nk=64, nb=4, a=10.0, z=8.0, dt=0.01)
 
 
 @pytest.fixture(scope="module")
-def cfg2():
+def ng64_nk64():
     S = po.System(C2["ng"], C2["nk"], C2["nb"], C2["nb"], C2["a"])
     vion = S.ion_potential([C2["z"]], [1.0], [0.0])
     O = po.Oracle(S, vion)
@@ -73,8 +74,8 @@ def cfg2():
     G.close()
 
 
-def test_cfg2_tables(cfg2):
-    S, O, G, _ = cfg2
+def test_ng64_nk64_tables(ng64_nk64):
+    S, O, G, _ = ng64_nk64
     vcg, vq = G.tables()
     assert rel(vcg, S.vCG()) < 1e-12
     full, nk = S.vCGkk(), S.nk
@@ -83,8 +84,8 @@ def test_cfg2_tables(cfg2):
         assert rel(vq[d + nk - 1], full[ikr * nk + ikc]) < 1e-12
 
 
-def test_cfg2_hpsi_and_observables(cfg2):
-    S, O, G, _ = cfg2
+def test_ng64_nk64_hpsi_and_observables(ng64_nk64):
+    S, O, G, _ = ng64_nk64
     u, w = random_orbitals(S, 201), random_orbitals(S, 202)
     for A in (0.0, 0.3):
         assert rel(G.ubkwbk_hwbk(u, w, A), O.ubkwbk_hwbk(u, w, A)) < 1e-12
@@ -95,10 +96,10 @@ def test_cfg2_hpsi_and_observables(cfg2):
         assert abs(j - O.ubk_Je(u, A)) < 1e-12 * max(1.0, abs(j))
 
 
-def test_cfg2_rk4_prefix_from_ground_state(cfg2):
+def test_ng64_nk64_rk4_prefix_from_ground_state(ng64_nk64):
     """16 steps of the benchmark pulse (A0=0.3, w=0.057, T=400) from step
     20000 (mid-pulse), starting from a GPU ground state."""
-    S, O, G, _ = cfg2
+    S, O, G, _ = ng64_nk64
     u0 = G.Gram_Schmidt(T.init_wf(C2["ng"], C2["nb"], C2["nk"], C2["a"]))
     ugs = G.GS_calculation(u0, 20, 4)[0]
     _, A, Aave, _ = T.define_afield_lp(20016, C2["dt"], *T.pulse_lp_params(0.3, 0.057, 400.0))
@@ -121,8 +122,8 @@ def test_cfg2_rk4_prefix_from_ground_state(cfg2):
     np.testing.assert_array_equal(G.download(), ug)
 
 
-def test_cfg2_advance_rejects_steps_beyond_the_field(cfg2):
-    S, O, G, _ = cfg2
+def test_ng64_nk64_advance_rejects_steps_beyond_the_field(ng64_nk64):
+    S, O, G, _ = ng64_nk64
     A = np.full(3, 0.01)
     G.upload(random_orbitals(S, 203))
     G.rt_begin(A, A)
@@ -136,37 +137,37 @@ def test_cfg2_advance_rejects_steps_beyond_the_field(cfg2):
 
 
 # --------------------------------------------------------------------------
-# cfg3: batch of NK in {16, 32, 64}, NG=64, NB=4
+# mixed_batch: batch of NK in {16, 32, 64}, NG=64, NB=4
 # --------------------------------------------------------------------------
-C3 = dict(ng=64, nb=4, a=10.0, z=8.0, dt=0.01, nt=30, nks=[16, 32, 64, 64, 32, 16])
+MIXED = dict(ng=64, nb=4, a=10.0, z=8.0, dt=0.01, nt=30, nks=[16, 32, 64, 64, 32, 16])
 
 
 @pytest.fixture(scope="module")
-def cfg3():
+def mixed_batch():
     us, vions, systems = [], [], []
-    for t, nk in enumerate(C3["nks"]):
-        S = po.System(C3["ng"], nk, C3["nb"], C3["nb"], C3["a"])
+    for t, nk in enumerate(MIXED["nks"]):
+        S = po.System(MIXED["ng"], nk, MIXED["nb"], MIXED["nb"], MIXED["a"])
         systems.append(S)
         us.append(random_orbitals(S, 300 + t))
-        vions.append(S.ion_potential([C3["z"]], [1.0], [0.0]))
-    A, Aave = seeded_pulses(len(C3["nks"]), 20000 + C3["nt"], C3["dt"], 400.0, 20261202)
+        vions.append(S.ion_potential([MIXED["z"]], [1.0], [0.0]))
+    A, Aave = seeded_pulses(len(MIXED["nks"]), 20000 + MIXED["nt"], MIXED["dt"], 400.0, 20261202)
     A, Aave = np.ascontiguousarray(A[:, 20000:]), np.ascontiguousarray(Aave[:, 20000:])
     singles = []
-    for t, nk in enumerate(C3["nks"]):
-        with T.TDHF(C3["ng"], nk, C3["nb"], C3["nb"], C3["a"], C3["dt"], vions[t]) as G:
+    for t, nk in enumerate(MIXED["nks"]):
+        with T.TDHF(MIXED["ng"], nk, MIXED["nb"], MIXED["nb"], MIXED["a"], MIXED["dt"], vions[t]) as G:
             singles.append(G.RT_calculation(us[t], A[t], Aave[t]))
     return systems, us, np.array(vions), A, Aave, singles
 
 
 @pytest.mark.parametrize("nw", [1, 2, 4, 8])
-def test_cfg3_batch_equals_single_runs(cfg3, nw):
-    systems, us, vions, A, Aave, singles = cfg3
-    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+def test_mixed_batch_batch_equals_single_runs(mixed_batch, nw):
+    systems, us, vions, A, Aave, singles = mixed_batch
+    with T.Batch(MIXED["ng"], MIXED["nb"], MIXED["nb"], MIXED["a"], MIXED["dt"], MIXED["nks"], vions) as B:
         B.set_waves(nw)
         B.upload(us)
         e, n, j, i3 = B.propagate(A, Aave)
         finals = B.download()
-    for t in range(len(C3["nks"])):
+    for t in range(len(MIXED["nks"])):
         u1, e1, n1, j1, i31 = singles[t]
         np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * np.abs(e1).max())
         np.testing.assert_allclose(n[t], n1, rtol=0, atol=1e-12)
@@ -176,15 +177,15 @@ def test_cfg3_batch_equals_single_runs(cfg3, nw):
 
 
 @pytest.mark.parametrize("sched", [1, 2])
-def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
-    systems, us, vions, A, Aave, singles = cfg3
-    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+def test_mixed_batch_batch_row_schedules_equal_single_runs(mixed_batch, sched):
+    systems, us, vions, A, Aave, singles = mixed_batch
+    with T.Batch(MIXED["ng"], MIXED["nb"], MIXED["nb"], MIXED["a"], MIXED["dt"], MIXED["nks"], vions) as B:
         B.set_schedule(sched)
         assert B.schedule() == sched
         B.upload(us)
         e, n, j, i3 = B.propagate(A, Aave)
         finals = B.download()
-    for t in range(len(C3["nks"])):
+    for t in range(len(MIXED["nks"])):
         u1, e1, n1, j1, i31 = singles[t]
         np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * np.abs(e1).max())
         np.testing.assert_allclose(j[t], j1, rtol=0, atol=1e-12 * max(1.0, np.abs(j1).max()))
@@ -193,14 +194,14 @@ def test_cfg3_batch_row_schedules_equal_single_runs(cfg3, sched):
 
 @pytest.mark.parametrize("mode,nw,sched", [(1, 1, 0), (1, 2, 1), (1, 8, 2), (2, 1, 2),
                                            (3, 1, 0), (3, 4, 2)])
-def test_cfg3_pair_tiles_equal_single_runs(cfg3, mode, nw, sched):
+def test_mixed_batch_pair_tiles_equal_single_runs(mixed_batch, mode, nw, sched):
     """The exchange from pair tiles (one transform pair per unordered pair of
     rows, Nyquist correction for the reversed pair) against the row-wise
     single runs, for NK = 16, 32, 64, with tiles of 64, 32 and 16 rows (modes
     1, 2, 3), the epilogue launch at 1 to 8 waves per workgroup, and tile
     lists pinned to XCDs per trajectory (row schedules 0, 1) or not (2)."""
-    systems, us, vions, A, Aave, singles = cfg3
-    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+    systems, us, vions, A, Aave, singles = mixed_batch
+    with T.Batch(MIXED["ng"], MIXED["nb"], MIXED["nb"], MIXED["a"], MIXED["dt"], MIXED["nks"], vions) as B:
         B.set_waves(nw)
         B.set_exchange(mode)
         B.set_schedule(sched)
@@ -208,7 +209,7 @@ def test_cfg3_pair_tiles_equal_single_runs(cfg3, mode, nw, sched):
         B.upload(us)
         e, n, j, i3 = B.propagate(A, Aave)
         finals = B.download()
-    for t in range(len(C3["nks"])):
+    for t in range(len(MIXED["nks"])):
         u1, e1, n1, j1, i31 = singles[t]
         np.testing.assert_allclose(e[t], e1, rtol=0, atol=1e-12 * np.abs(e1).max())
         np.testing.assert_allclose(n[t], n1, rtol=0, atol=1e-12)
@@ -256,29 +257,29 @@ def test_pair_tiles_need_full_occupation():
         assert B.exchange() == 0
 
 
-def test_cfg3_batch_matches_oracle(cfg3):
+def test_mixed_batch_batch_matches_oracle(mixed_batch):
     """One trajectory of each k-mesh of the batch against the oracle, over an
     8-step prefix."""
-    systems, us, vions, A, Aave, _ = cfg3
-    with T.Batch(C3["ng"], C3["nb"], C3["nb"], C3["a"], C3["dt"], C3["nks"], vions) as B:
+    systems, us, vions, A, Aave, _ = mixed_batch
+    with T.Batch(MIXED["ng"], MIXED["nb"], MIXED["nb"], MIXED["a"], MIXED["dt"], MIXED["nks"], vions) as B:
         B.upload(us)
         e, n, j, _ = B.propagate(A[:, :8], Aave[:, :8])
     for t in (0, 1, 2):  # NK = 16, 32, 64
         O = po.Oracle(systems[t], vions[t])
-        _, eo, no, jo, _ = O.RT_calculation(us[t], C3["dt"], A[t, :8], Aave[t, :8])
+        _, eo, no, jo, _ = O.RT_calculation(us[t], MIXED["dt"], A[t, :8], Aave[t, :8])
         assert np.abs(e[t] - eo).max() < 1e-10
         assert np.abs(n[t] - no).max() < 1e-10
         assert rel(j[t], jo) < 1e-8
 
 
 # --------------------------------------------------------------------------
-# cfg4: adjoint, a=8, NG=32, NK=32, NB=2, Z=4, 4000 steps dt=0.025
+# adjoint_ng32: adjoint, a=8, NG=32, NK=32, NB=2, Z=4, 4000 steps dt=0.025
 # --------------------------------------------------------------------------
 C4 = dict(ng=32, nk=32, nb=2, a=8.0, z=4.0, dt=0.025, nt=4000, nknots=200, sigma=0.05)
 
 
 @pytest.fixture(scope="module")
-def cfg4():
+def adjoint_ng32():
     S = po.System(C4["ng"], C4["nk"], C4["nb"], C4["nb"], C4["a"])
     vion = S.ion_potential([C4["z"]], [1.0], [0.0])
     G = T.TDHF(C4["ng"], C4["nk"], C4["nb"], C4["nb"], C4["a"], C4["dt"], vion)
@@ -291,10 +292,10 @@ def cfg4():
     G.close()
 
 
-def test_cfg4_adjoint_matches_oracle_mid_pulse(cfg4):
+def test_adjoint_ng32_adjoint_matches_oracle_mid_pulse(adjoint_ng32):
     """The gradient of E(T) - E_GS over 40 steps from the state at step 2000
     of the pulse, against the oracle's adjoint of the same steps."""
-    S, G, vion, ugs, egs, P, knots, _ = cfg4
+    S, G, vion, ugs, egs, P, knots, _ = adjoint_ng32
     A = P @ knots
     Aave = 0.5 * (A + np.concatenate([[0.0], A[:-1]]))
     G.upload(ugs)
@@ -314,10 +315,10 @@ def test_cfg4_adjoint_matches_oracle_mid_pulse(cfg4):
     assert np.abs(gu - guo).max() < 1e-9 * np.abs(guo).max()
 
 
-def test_cfg4_full_run_finite_differences(cfg4):
+def test_adjoint_ng32_full_run_finite_differences(adjoint_ng32):
     """The full 4000-step gradient along 8 random knot directions against
     central differences of the forward objective (h = 1e-5)."""
-    S, G, vion, ugs, egs, P, knots, rng = cfg4
+    S, G, vion, ugs, egs, P, knots, rng = adjoint_ng32
     L, _, e, _, _ = G.fwd(ugs, P @ knots, eref=egs)
     gA, _ = G.bwd(want_u0=False)
     gk = P.T @ gA
@@ -333,17 +334,17 @@ def test_cfg4_full_run_finite_differences(cfg4):
         worst = max(worst, abs(fd - gk @ d) / (1e-6 * abs(fd) + floor))
         assert abs(fd - gk @ d) <= 1e-6 * abs(fd) + floor, (fd, gk @ d, floor)
     assert np.isfinite(e).all() and L > 0  # the pulse excites the system
-    print(f"cfg4 FD: worst |fd - g.d| / (1e-6 |fd| + floor) = {worst:.3f}, floor {floor:.2e}")
+    print(f"adjoint_ng32 FD: worst |fd - g.d| / (1e-6 |fd| + floor) = {worst:.3f}, floor {floor:.2e}")
 
 
 # --------------------------------------------------------------------------
-# cfg5: a=12, NG=128, NK=128, NB=NBocc=8, Z=16
+# ng128_nk128: a=12, NG=128, NK=128, NB=NBocc=8, Z=16
 # --------------------------------------------------------------------------
 C5 = dict(ng=128, nk=128, nb=8, a=12.0, z=16.0, dt=0.004)
 
 
 @pytest.fixture(scope="module")
-def cfg5():
+def ng128_nk128():
     S = po.System(C5["ng"], C5["nk"], C5["nb"], C5["nb"], C5["a"])
     vion = S.ion_potential([C5["z"]], [1.0], [0.0])
     G = T.TDHF(C5["ng"], C5["nk"], C5["nb"], C5["nb"], C5["a"], C5["dt"], vion)
@@ -351,8 +352,8 @@ def cfg5():
     G.close()
 
 
-def test_cfg5_tables_and_hpsi(cfg5):
-    S, G, vion = cfg5
+def test_ng128_nk128_tables_and_hpsi(ng128_nk128):
+    S, G, vion = ng128_nk128
     vcg, vq = G.tables()
     assert rel(vcg, S.vCG()) < 1e-12
     full, nk = S.vCGkk(), S.nk
@@ -368,11 +369,11 @@ def test_cfg5_tables_and_hpsi(cfg5):
     assert abs(j - O.ubk_Je(u, -0.1)) < 1e-12 * max(1.0, abs(j))
 
 
-def test_cfg5_jsq_gradient_finite_differences(cfg5):
-    """Three cfg3-style pulses in one launch, 4 steps from mid-pulse,
+def test_ng128_nk128_jsq_gradient_finite_differences(ng128_nk128):
+    """Three mixed_batch-style pulses in one launch, 4 steps from mid-pulse,
     objective dt sum J^2: each pulse's gradient against central differences
     along a random direction (one pair of batched forwards)."""
-    S, G, vion = cfg5
+    S, G, vion = ng128_nk128
     u0 = random_orbitals(S, 503)
     A, _ = seeded_pulses(3, 50004, C5["dt"], 400.0, 20261205)
     A = np.ascontiguousarray(A[:, 50000:])

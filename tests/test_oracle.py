@@ -340,30 +340,30 @@ def test_krylov_step_invariant_under_band_rotation():
 
 
 @pytest.fixture(scope="module")
-def gs_cfg1():
+def gs_ng32_nk8():
     S, O, vion = make(32, 8, 2, 2, 8.0, 4.0)
     u0 = S.gram_schmidt(S.init_wf())
     u, eh, eps, err = O.GS_calculation(u0, 250, 4)
     return S, O, u, eh, eps, err
 
 
-def test_gs_converged(gs_cfg1):
-    S, O, u, eh, eps, err = gs_cfg1
+def test_gs_converged(gs_ng32_nk8):
+    S, O, u, eh, eps, err = gs_ng32_nk8
     assert abs(eh[-1] - eh[-2]) < 1e-12
     assert err.max() < 1e-5
     assert abs(O.ubk_Je(u, 0.0)) < 1e-10  # no current in the ground state
 
 
-def test_gs_eigenvalues_independent_of_ncg(gs_cfg1):
-    S, O, u, eh, eps, err = gs_cfg1
+def test_gs_eigenvalues_independent_of_ncg(gs_ng32_nk8):
+    S, O, u, eh, eps, err = gs_ng32_nk8
     u0 = S.gram_schmidt(S.init_wf())
     _, eh5, eps5, _ = O.GS_calculation(u0, 250, 5)
     assert abs(eh5[-1] - eh[-1]) < 1e-10
     np.testing.assert_allclose(eps5, eps, rtol=0, atol=1e-9)
 
 
-def test_rk4_fourth_order(gs_cfg1):
-    S, O, u, *_ = gs_cfg1
+def test_rk4_fourth_order(gs_ng32_nk8):
+    S, O, u, *_ = gs_ng32_nk8
     up = u.copy(order="F")
     up[:, 0, :] *= np.exp(0.3j * np.sin(2 * np.pi * np.arange(32) / 32))[:, None]  # excite
     e0 = O.ubk_Etot(up, 0.0)
