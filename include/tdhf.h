@@ -298,8 +298,9 @@ int tdhf_batch_propagate(tdhf_batch* b, int nt, const double* A, const double* A
 int tdhf_batch_run(const tdhf_params* p, int ntraj, const int* nk, const double* vion,
                    double* u, int nt, const double* A, const double* Aave, double* etot,
                    double* nele, double* jtot, double* init3);
-/* Launch shape: waves per workgroup of the fused kernel (1, 2, 4 or 8, at
- * least NG/64; default the smallest).
+/* Launch shape: waves per workgroup of the fused kernel, or with pair tiles of
+ * the per-row epilogue launch (1, 2, 4 or 8, at least NG/64; default the
+ * smallest).
  * tdhf_batch_autotune times nt_trial steps for each candidate on a copy of the
  * state (the batch state is unchanged), keeps the fastest and remembers it for
  * batches of the same shape created later in the process; best_nw receives
@@ -324,8 +325,9 @@ int tdhf_batch_get_schedule(const tdhf_batch* b);
  * (k,i),(q,j) and (q,j),(k,i) are conjugates up to the Nyquist component, so
  * one transform pair per unordered pair feeds both rows (needs NG = 64,
  * NB == NBocc and fewer than 2^22 rows per trajectory; TDHF_EINVAL
- * otherwise).  Smaller tiles make more workgroups and more partial sums per
- * row.  Results agree to rounding.  tdhf_batch_autotune includes all four. */
+ * otherwise); a second launch per stage adds each row's partial sums and
+ * applies the rest of H and the stage update.  Smaller tiles make more
+ * workgroups and more partial sums per row.  Results agree to rounding.  tdhf_batch_autotune includes all four. */
 int tdhf_batch_set_exchange(tdhf_batch* b, int mode);
 int tdhf_batch_get_exchange(const tdhf_batch* b);
 int tdhf_batch_set_kernel_timing(tdhf_batch* b, int enable);

@@ -62,6 +62,7 @@ struct tdhf_ctx {
   double *d_kx = nullptr, *d_vion = nullptr, *d_vcg = nullptr;
   double *d_vq = nullptr, *d_vtl = nullptr;
   double* d_kc = nullptr;  // circulant kernels K_H, K_2, K_1 of the apply kernel
+  double* d_vhg = nullptr; // vCG/NG in the transforms' G layout (pair-tile epilogue)
   zd* d_tw = nullptr;
   // orbital buffers (NG*NB*NK complex each)
   zd *d_u0 = nullptr, *d_ksum = nullptr, *d_sa = nullptr, *d_sb = nullptr;
@@ -183,19 +184,13 @@ void dfree(T*& p) {
 }
 
 // Apply-kernel dispatch over the supported grid sizes.
-template <int NG>
-size_t lds_bytes(int nw, bool epilogue_only) {
-  return tdhf::apply_lds_bytes<NG>(nw, epilogue_only);
-}
-
-// LDS of an apply_kernel launch; epilogue_only: the exchange comes from the
-// pair tiles (no transpose tiles needed)
-size_t lds_for(int ng, int nw, bool epilogue_only = false) {
+// LDS of an apply_kernel launch
+size_t lds_for(int ng, int nw) {
   switch (ng) {
-    case 16: return lds_bytes<16>(nw, epilogue_only);
-    case 32: return lds_bytes<32>(nw, epilogue_only);
-    case 64: return lds_bytes<64>(nw, epilogue_only);
-    case 128: return lds_bytes<128>(nw, epilogue_only);
+    case 16: return tdhf::apply_lds_bytes<16>(nw);
+    case 32: return tdhf::apply_lds_bytes<32>(nw);
+    case 64: return tdhf::apply_lds_bytes<64>(nw);
+    case 128: return tdhf::apply_lds_bytes<128>(nw);
   }
   return 0;
 }
@@ -224,13 +219,44 @@ int device_cus() {
   return n;
 }
 
-template <int NG, bool XP = false>
+template <int NG>
 void launch_apply_t(const tdhf::ApplyArgs& a, int rows, int ntraj, int nw, size_t lds,
                     hipStream_t s) {
   tdhf::ApplyArgs b = a;
   // wave-priority balancing pays only when the grid is one workgroup per CU
   b.balance = (long)rows * ntraj <= device_cus() ? 1 : 0;
-  hipLaunchKernelGGL((tdhf::apply_kernel<NG, XP>), dim3(rows, ntraj), dim3(64 * nw), lds, s, b);
+  if constexpr (NG >= 64) {
+    if (tdhf::apply_by_transforms<NG>(nw)) {
+      hipLaunchKernelGGL((tdhf::apply_kernel<NG, true>), dim3(rows, ntraj), dim3(64 * nw), lds, s,
+                         b);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((tdhf::apply_kernel<NG, false>), dim3(rows, ntraj), dim3(64 * nw), lds, s, b);
+}
+
+// Raises the dynamic LDS limit of every apply_kernel variant of grid size ng
+// to the largest launch any context or batch makes (8 waves).  The limit is a
+// property of the kernel, shared by every context of the process, so it is
+// never lowered below what another one uses.
+hipError_t set_apply_lds_limit(int ng) {
+  const int cap = (int)lds_for(ng, 8);
+  auto set = [cap](const void* fn) {
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  };
+  switch (ng) {
+    case 16: return set((const void*)tdhf::apply_kernel<16, false>);
+    case 32: return set((const void*)tdhf::apply_kernel<32, false>);
+    case 64: {
+      const hipError_t e = set((const void*)tdhf::apply_kernel<64, false>);
+      return e != hipSuccess ? e : set((const void*)tdhf::apply_kernel<64, true>);
+    }
+    case 128: {
+      const hipError_t e = set((const void*)tdhf::apply_kernel<128, false>);
+      return e != hipSuccess ? e : set((const void*)tdhf::apply_kernel<128, true>);
+    }
+  }
+  return hipErrorInvalidValue;
 }
 
 // rows workgroups per trajectory; ntraj > 1 for a uniform batch (grid y)
@@ -301,6 +327,7 @@ tdhf::ApplyArgs base_args(const tdhf_ctx* c) {
   a.vion = c->d_vion;
   a.vtl = c->d_vtl;
   a.kc = c->d_kc;
+  a.vhg = c->d_vhg;
   a.tw = c->d_tw;
   a.obs_stride = c->obs_stride;
   return a;
@@ -614,6 +641,8 @@ int build_tables(tdhf_ctx* c, const double* vcg, const double* vcgkk) {
     HIPCHK(c, hipMemcpy(c->d_kc, kc.data(), kc.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   CHKRET(relayout(c, c->d_vq, ND, -1.0 / ((double)NG * (double)NK), c->d_vtl));
+  CHKRET(dalloc(c, &c->d_vhg, NG));
+  CHKRET(relayout(c, c->d_vcg, 1, 1.0 / (double)NG, c->d_vhg));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -623,7 +652,7 @@ void release(tdhf_ctx* c) {
   for (auto e : c->ev_pool) (void)hipEventDestroy(e);
   c->ev_pool.clear();
   dfree(c->d_kx); dfree(c->d_vion); dfree(c->d_vcg);
-  dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw); dfree(c->d_kc);
+  dfree(c->d_vq); dfree(c->d_vtl); dfree(c->d_tw); dfree(c->d_kc); dfree(c->d_vhg);
   dfree(c->d_u0); dfree(c->d_ksum); dfree(c->d_sa); dfree(c->d_sb);
   dfree(c->d_w); dfree(c->d_out);
   dfree(c->d_obs); dfree(c->d_res); dfree(c->d_A); dfree(c->d_Aave); dfree(c->d_step);
@@ -720,17 +749,10 @@ int tdhf_create(const tdhf_params* p, const double* vion, const double* vcg, con
   }
   if ((rc = build_tables(c, vcg, vcgkk))) return bail(rc);
   {
-    const void* fn = c->ng == 16   ? (const void*)tdhf::apply_kernel<16>
-                     : c->ng == 32 ? (const void*)tdhf::apply_kernel<32>
-                     : c->ng == 64 ? (const void*)tdhf::apply_kernel<64>
-                                   : (const void*)tdhf::apply_kernel<128>;
-    // the limit is a property of the kernel, shared by every context of the
-    // process: set it to the largest launch any context makes (8 waves), so
-    // a context created later never lowers it below what another one uses
-    const size_t cap = lds_for(c->ng, 8);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)cap);
+    const hipError_t e = set_apply_lds_limit(c->ng);
     if (e != hipSuccess && c->lds > 64 * 1024) {
-      fail(c, TDHF_EDEVICE, "cannot raise LDS limit to %zu bytes: %s", cap, hipGetErrorString(e));
+      fail(c, TDHF_EDEVICE, "cannot raise LDS limit to %zu bytes: %s", lds_for(c->ng, 8),
+           hipGetErrorString(e));
       return bail(TDHF_EDEVICE);
     }
   }

@@ -8,8 +8,9 @@
 // trajectory to one XCD (workgroups are dispatched round-robin over the 8
 // XCDs, so entry b runs on XCD b % 8) so that its orbitals stay in that XCD's
 // L2.  The exchange is evaluated row-wise by that kernel or, for NG = 64 with
-// every band occupied, by pair tiles (tdhf_pairtile.h) whose per-row partial
-// sums the kernel then adds.  The RK4 steps are captured as a hipGraph and
+// every band occupied, by pair tiles (tdhf_pairtile.h), whose per-row partial
+// sums pair_epilogue_kernel then adds before the rest of H and the stage
+// update (two launches per stage).  The RK4 steps are captured as a hipGraph and
 // replayed; the exchange evaluation and tile size, the row schedule and the
 // launch shape (waves per workgroup) are picked by tdhf_batch_autotune.
 #pragma once
@@ -51,7 +52,7 @@ struct tdhf_batch {
   int nw = 4;
   int sched = 0;  // row schedule (build_schedule)
   // exchange: 0 = row-wise in the fused kernel; 1, 2, 3 = pair tiles of 64,
-  // 32, 16 rows (pairtile_kernel<8, 4, 2>) + partial sums in the fused kernel
+  // 32, 16 rows (pairtile_kernel<8, 4, 2>) + pair_epilogue_kernel
   int xmode = 0;
   int graph_xmode = 0;
   int xrows = 0;      // tile rows the partial buffers and the tile list are laid out for
@@ -59,6 +60,8 @@ struct tdhf_batch {
   double* d_dpart = nullptr;
   int4* d_tiles = nullptr;
   int ntiles = 0;
+  int2* d_erows = nullptr;  // (trajectory, row) of every row, for pair_epilogue_kernel
+  int nerows = 0;
   size_t tiles_cap = 0;  // entries d_tiles holds
   std::vector<size_t> xoff, doff;  // per trajectory (complex / double elements)
   size_t zoff = 0;                 // NG zeros in d_dpart after the partials
@@ -295,10 +298,22 @@ int batch_set_xmode(tdhf_batch* b, int xmode) {
     BHIP(b, hipFuncSetAttribute((const void*)tdhf::pairtile_kernel<2>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)tdhf::pairtile_lds_bytes(2)));
-    BHIP(b, hipFuncSetAttribute((const void*)tdhf::apply_kernel<64, true>,
+    BHIP(b, hipFuncSetAttribute((const void*)tdhf::pair_epilogue_kernel,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds_for(64, 8, true)));
+                                (int)tdhf::pair_epilogue_lds_bytes(8)));
     b->xrows = TR;
+  }
+  if (xmode && !b->d_erows) {
+    // the epilogue's rows, trajectory by trajectory (a wave's 8 rows mostly
+    // share a trajectory's tables and density partials)
+    std::vector<int2> er;
+    for (int t = 0; t < b->ntraj; ++t)
+      for (int r = 0; r < b->p.nb * b->nk[t]; ++r) er.push_back(make_int2(t, r));
+    CHKRET(balloc(b, &b->d_erows, er.size()));
+    BHIP(b, hipMemcpyAsync(b->d_erows, er.data(), er.size() * sizeof(int2), hipMemcpyHostToDevice,
+                           b->stream));
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->nerows = (int)er.size();
   }
   b->xmode = xmode;
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -346,6 +361,7 @@ int batch_update_desc(tdhf_batch* b) {
     d.nblk = b->xmode ? (rows + b->xrows - 1) / b->xrows : 0;
     d.xpart = b->xmode ? b->d_xpart + b->xoff[t] : nullptr;
     d.dpart = b->xmode ? b->d_dpart + b->doff[t] : nullptr;
+    d.vhg = c->d_vhg;
   }
   BHIP(b, hipMemcpyAsync(b->d_td, b->h_td.data(), b->h_td.size() * sizeof(tdhf::TrajDesc),
                          hipMemcpyHostToDevice, b->stream));
@@ -354,7 +370,7 @@ int batch_update_desc(tdhf_batch* b) {
 }
 
 int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
-  // with timing on, one event pair brackets the stage (pair tiles + fused kernel)
+  // with timing on, one event pair brackets the stage (pair tiles + epilogue)
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (b->timing) {
     if (b->ev_used + 2 > b->ev.size())
@@ -391,10 +407,13 @@ int batch_launch(tdhf_batch* b, const tdhf::ApplyArgs& a, int nw) {
     }
     BHIP(b, hipGetLastError());
   }
-  const size_t lds = lds_for(b->p.ng, nw, b->xmode != 0);
-  if (b->xmode) {  // pair tiles: NG = 64 (batch_pairtile_ok)
-    launch_apply_t<64, true>(a, b->ngrid, 1, nw, lds, b->stream);
+  if (b->xmode) {  // pair tiles: NG = 64 (batch_pairtile_ok); 8 rows per wave
+    const int rpb = 8 * nw;
+    hipLaunchKernelGGL(tdhf::pair_epilogue_kernel, dim3((b->nerows + rpb - 1) / rpb),
+                       dim3(64 * nw), tdhf::pair_epilogue_lds_bytes(nw), b->stream, a, b->d_erows,
+                       b->nerows);
   } else {
+    const size_t lds = lds_for(b->p.ng, nw);
     switch (b->p.ng) {
       case 16: launch_apply_t<16>(a, b->ngrid, 1, nw, lds, b->stream); break;
       case 32: launch_apply_t<32>(a, b->ngrid, 1, nw, lds, b->stream); break;
@@ -518,7 +537,7 @@ void batch_release(tdhf_batch* b) {
   b->gexec = nullptr;
   for (auto e : b->ev) (void)hipEventDestroy(e);
   b->ev.clear();
-  dfree(b->d_vion); dfree(b->d_state); dfree(b->d_td); dfree(b->d_rowmap);
+  dfree(b->d_vion); dfree(b->d_state); dfree(b->d_td); dfree(b->d_rowmap); dfree(b->d_erows);
   dfree(b->d_A); dfree(b->d_Aave); dfree(b->d_obs); dfree(b->d_res); dfree(b->d_step);
   dfree(b->d_xpart); dfree(b->d_dpart); dfree(b->d_tiles);
   for (auto c : b->tabs) tdhf_destroy(c);
@@ -636,14 +655,7 @@ int tdhf_batch_create(const tdhf_params* p, int ntraj, const int* nk, const doub
   b->h_td.resize(ntraj);
   if ((rc = batch_update_desc(b))) return bail(rc);
   if (cached && ct[2] && (rc = batch_set_xmode(b, ct[2]))) return bail(rc);
-  {
-    const void* fn = p->ng == 16   ? (const void*)tdhf::apply_kernel<16>
-                     : p->ng == 32 ? (const void*)tdhf::apply_kernel<32>
-                     : p->ng == 64 ? (const void*)tdhf::apply_kernel<64>
-                                   : (const void*)tdhf::apply_kernel<128>;
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              (int)lds_for(p->ng, 8));
-  }
+  (void)set_apply_lds_limit(p->ng);
   *out = b;
   return 0;
 }

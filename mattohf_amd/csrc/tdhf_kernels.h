@@ -10,18 +10,23 @@
 // accumulates the density sum_{q,j} |u(q,j)|^2 (ubk_dns, src/Functions.f90
 // :55-71), so no separate density kernel is needed.  After a workgroup-wide
 // reduction, the Hartree round trip (dns_vH, :147-162) and the kinetic round
-// trip of w (ubk_tubk, :212-234) are evaluated as the circulant products they
-// are, vH = K_H * rho and T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w, with
+// trip of w (ubk_tubk, :212-234) are evaluated either as the circulant
+// products they are, vH = K_H * rho and T w = K_2 * w + (k+A) K_1 * w +
+// (k+A)^2/2 w, with
 //     K_H(d) = 1/NG sum_G vCG(G) cos(2 pi g d/NG),
 //     K_2(d) = 1/NG sum_G G^2/2 cos(2 pi g d/NG),
 //     K_1(d) = 1/NG sum_G G exp(2 pi i g d/NG)
-// (G in FFT order, Nyquist at -NG/2 b), by all threads of the workgroup
-// instead of one wave's serial transforms.  The epilogue forms
+// (G in FFT order, Nyquist at -NG/2 b), by all threads of the workgroup, or,
+// when that would leave each thread 32 terms or more (workgroups of one or
+// two waves, NG = 128), as one transform round trip each,
+//     vH = occ IFFT[vCG/NG . FFT[rho]],   T w = IFFT[(G+k+A)^2/(2 NG) . FFT[w]],
+// on two groups of the first wave.  The epilogue forms
 // H w = T w + (vH + vion) w + V_F w (ubkwbk_hwbk, :187-209) and applies the
 // mode-specific update (plain H|w>, one RK4 stage of RT_RK4_ubk,
 // src/RT_calculation.f90:154-173, or observables only).  Per-row partial sums
-// of Ekin, J and Ex (src/Functions.f90:94-119,165-183,289-318) follow from the
-// same products by Parseval: sum_G f(G)|w_G|^2/NG = sum_x conj(w) (K_f * w).
+// of Ekin and J (src/Functions.f90:94-119,289-318) follow by Parseval, from
+// the circulant products, sum_x conj(w) (K_f * w) = sum_G f(G)|w_G|^2/NG, or
+// from FFT[w]; Ex (:165-183) from V_F w.
 #pragma once
 #include "tdhf_fft.h"
 
@@ -53,12 +58,13 @@ struct TrajDesc {
   const double* A;
   const double* Aave;
   double* obs;
-  // exchange from the pair tiles (tdhf_pairtile.h): nblk partial sums per
-  // row (row r, block c at (r * nblk + c) * NG) and nblk density partials
-  // (block b at b * NG); nullptr: the row-wise exchange of apply_kernel
+  // exchange from the pair tiles (tdhf_pairtile.h, read by
+  // pair_epilogue_kernel): nblk partial sums per row (row r, block c at
+  // (r * nblk + c) * NG) and nblk density partials (block b at b * NG)
   zd* xpart;
   double* dpart;
   int nblk;
+  const double* vhg;  // vCG(G)/NG in the G layout of GroupFFT<NG> (pair-tile epilogue)
 };
 
 // Offsets inside one observables slot: rows*3 partials then (N, Eion, EH).
@@ -76,6 +82,7 @@ struct ApplyArgs {
   const double* vion;       // NG
   const double* vtl;        // (2NK-1)*NG, G layout, -v(G+k-q)/(NG*NK)
   const double* kc;         // 4*NG: circulant kernels K_H, K_2, K_1 (complex)
+  const double* vhg;        // NG: vCG(G)/NG in the G layout of GroupFFT<NG>
   const zd* tw;             // twiddle table of GroupFFT<NG> (2*NG)
   const zd* u;              // (NG,NB,NK) orbitals defining rho and V_F
   const zd* w;              // (NG,nbw,NK) orbitals H acts on
@@ -192,8 +199,8 @@ __host__ __device__ constexpr size_t apply_part_doubles(int nw, int NG) {
 
 // Per-group LDS slot of apply_kernel (complex elements): the transpose tile
 // during the pair loop; afterwards the slot area holds the wave partials of
-// the reduction (3 NG doubles per wave), the circulant-product partials and
-// the observables staging.  At least 3 NG / 2 elements, rounded up to 8 mod 16
+// the reduction (3 NG doubles per wave), the circulant-product partials or the
+// tiles of the Hartree and kinetic transforms, and the observables staging.  At least 3 NG / 2 elements, rounded up to 8 mod 16
 // elements, so that consecutive groups start half a bank cycle (32 of the 64
 // four-byte banks) apart and the transposes of two groups served in one LDS
 // pass do not collide.
@@ -203,30 +210,26 @@ __host__ __device__ constexpr int apply_slot() {
   return need + ((8 - need % 16) + 16) % 16;
 }
 
-// Slot of a launch that takes its exchange from the pair tiles (no pair loop,
-// no transposes): just enough for the epilogue scratch at the smallest launch.
-template <int NG>
-__host__ __device__ constexpr int apply_epi_slot() {
-  constexpr int nwmin = NG > 64 ? NG / 64 : 1, gpw = 64 / GroupFFT<NG>::L;
-  constexpr int need = (5 * 64 * nwmin + 7 * NG + 2 * gpw * nwmin - 1) / (2 * gpw * nwmin);
-  return need + ((8 - need % 16) + 16) % 16;
-}
-
 // LDS footprint of apply_kernel<NG> with nw waves (bytes): the groups' slots
 // (also the circulant-product partials and observables staging after the
-// pair loop), then w, V_F w (followed by rho), K_1 (complex NG each), rho,
-// K_H, K_2 (NG doubles each), a small scratch and the reduction scratch.
+// pair loop), then w, V_F w (followed by rho), K_1 or T w (complex NG each),
+// rho, K_H or vH, K_2 or vCG/NG in the G layout (NG doubles each), a small
+// scratch and the reduction scratch.
 template <int NG>
-__host__ __device__ constexpr size_t apply_lds_bytes(int nw, bool epilogue_only = false) {
-  return (size_t)nw * (64 / GroupFFT<NG>::L) *
-             (epilogue_only ? apply_epi_slot<NG>() : apply_slot<NG>()) * sizeof(zd) +
+__host__ __device__ constexpr size_t apply_lds_bytes(int nw) {
+  return (size_t)nw * (64 / GroupFFT<NG>::L) * apply_slot<NG>() * sizeof(zd) +
          3 * NG * sizeof(zd) + 3 * NG * sizeof(double) + 16 * sizeof(double) +
          apply_part_doubles(nw, NG) * sizeof(double);
 }
 
-// XP: the exchange and the density come from the pair tiles' partial sums
-// (TrajDesc.xpart; batched launches only): the pair loop is compiled out.
-template <int NG, bool XP = false>
+// TE: the Hartree and kinetic round trips as transforms (else circulant
+// products); the launcher chooses (apply_by_transforms).
+template <int NG>
+__host__ __device__ constexpr bool apply_by_transforms(int nw) {
+  return NG >= 64 && NG >= 32 * (64 * nw / NG);
+}
+
+template <int NG, bool TE>
 __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   using F = GroupFFT<NG>;
   constexpr int R = F::R, L = F::L, GPW = 64 / L;
@@ -237,9 +240,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
                     2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
                 "slot area too small for the epilogue scratch");
-  static_assert(5 * 64 * (NG > 64 ? NG / 64 : 1) + 7 * NG <=
-                    2 * apply_epi_slot<NG>() * GPW * (NG > 64 ? NG / 64 : 1),
-                "epilogue-only slot too small for the epilogue scratch");
   extern __shared__ zd smem[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nw = blockDim.x >> 6;
@@ -249,14 +249,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   int row = blockIdx.x;
   int nk = a.nk, obs_stride = a.obs_stride;
   double occ = a.occ;
-  const double *kx = a.kx, *vion = a.vion, *vtl = a.vtl, *kc = a.kc;
+  const double *kx = a.kx, *vion = a.vion, *vtl = a.vtl, *kc = a.kc, *vhg = a.vhg;
   const double *Aarr = a.Aarr, *Aavearr = a.Aavearr;
   const zd *u = a.u, *w = a.w;
   zd *u0 = a.u0, *ksum = a.ksum, *unext = a.unext, *out = a.out;
   double* obs = a.obs;
-  const zd* xpart = nullptr;        // exchange partials of the pair tiles
-  const double* dpart = nullptr;
-  int nblk = 0;
   const int tr = blockIdx.y;  // trajectory of a uniform batch (0 otherwise)
   if (tr > 0) {
     Aarr += (size_t)tr * a.fstride;
@@ -274,14 +271,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     vion = t.vion;
     vtl = t.vtl;
     kc = t.kc;
+    vhg = t.vhg;
     Aarr = t.A;
     Aavearr = t.Aave;
     u0 = t.u0;
     ksum = t.ksum;
     obs = t.obs;
-    xpart = t.xpart;
-    dpart = t.dpart;
-    nblk = t.nblk;
     switch (a.mode) {
       case M_S1: u = t.u0; unext = t.sa; break;
       case M_S2: u = t.sa; unext = t.sb; break;
@@ -293,15 +288,14 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   }
   const int ik = row / a.nbw, ib = row - ik * a.nbw;
 
-  // exchange from the pair tiles: no transposes, only the epilogue scratch
-  constexpr int ssz = XP ? apply_epi_slot<NG>() : SSZ;
-  zd* T = smem + grp * ssz;
-  zd* sW = smem + ngrp * ssz;
+  zd* T = smem + grp * SSZ;
+  zd* sW = smem + ngrp * SSZ;
   zd* sVF = sW + NG;
   double* sRho = reinterpret_cast<double*>(sVF + NG);  // follows V_F w (one reduction)
-  zd* sK1 = reinterpret_cast<zd*>(sRho + NG);
-  double* sKH = reinterpret_cast<double*>(sK1 + NG);
-  double* sK2 = sKH + NG;
+  const int P = blockDim.x / NG;  // >= 1: at least NG threads
+  zd* sK1 = reinterpret_cast<zd*>(sRho + NG);          // K_1, or T w
+  double* sKH = reinterpret_cast<double*>(sK1 + NG);   // K_H, or vH
+  double* sK2 = sKH + NG;                              // K_2, or vCG/NG (G layout)
   double* sRed = sK2 + NG;
   double* sPart = sRed + 16;  // apply_part_doubles(nw, NG)
 
@@ -336,7 +330,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // which go to LDS ----
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   F fft;
-  if constexpr (!XP) fft.init(a.tw, l);
+  fft.init(a.tw, l);
 
   zd acc[R], un[R];
   double den[R];
@@ -345,7 +339,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   // u(q,j) of the next pair is fetched while the current pair is transformed;
   // the multipliers of the current pair are loaded before that prefetch, so
   // waiting for them leaves the prefetch in flight.
-  const int nitem = XP ? 0 : nk * a.nbocc;  // pair tiles: no pairs here
+  const int nitem = nk * a.nbocc;
   // item = jb + nbocc iq; a group's items are ngrp apart
   const int dq = ngrp / a.nbocc, dj = ngrp - dq * a.nbocc;
   int cq = grp / a.nbocc, cj = grp - cq * a.nbocc;  // item of the current pass
@@ -356,22 +350,16 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       un[m] = gload(urow + L * m + l);
     });
   };
-  if (!XP && grp < nitem) fetch(cq, cj);
+  if (grp < nitem) fetch(cq, cj);
   zd w_o = mk(0.0, 0.0), k1_o = mk(0.0, 0.0);
   double k2_o = 0.0;
   if (tid < NG) {
     w_o = gload(w + o);
-    k2_o = gload(kc + NG + tid);
-    k1_o = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
-  }
-  // exchange and density from the pair tiles' partials, added in block order
-  // (loaded with the other operands; used after the empty pair loop)
-  zd xvf = mk(0.0, 0.0);
-  double xrho = 0.0;
-  if (XP && tid < NG) {
-    for (int c = 0; c < nblk; ++c) {
-      xvf = zadd(xvf, gload(xpart + ((size_t)row * nblk + c) * NG + tid));
-      xrho += gload(dpart + (size_t)c * NG + tid);
+    if constexpr (TE) {
+      k2_o = gload(vhg + tid);
+    } else {
+      k2_o = gload(kc + NG + tid);
+      k1_o = gload(reinterpret_cast<const zd*>(kc + 2 * NG) + tid);
     }
   }
   const double kxv = gload(kx + ik);
@@ -397,7 +385,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   lds_barrier();  // sW visible
 
   int it = 0;
-  if constexpr (!XP)
   for (int item = grp; item < nitem; item += ngrp, ++it) {
     if (a.balance) set_wave_priority(3 - (4 * it) / (wave_items > 0 ? wave_items : 1));
     zd uq[R], v[R];
@@ -474,7 +461,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   zd u0v = mk(0.0, 0.0), ksv = mk(0.0, 0.0);
   double vionv = 0.0, khv = 0.0;
   if (tid < NG) {
-    khv = gload(kc + tid);
+    if constexpr (!TE) khv = gload(kc + tid);
     vionv = gload(vion + tid);
     if (a.mode >= M_S1 && a.mode <= M_S4) {
       u0v = gload(u0 + o);
@@ -482,12 +469,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
     }
   }
 
-  if constexpr (XP) {
-    if (tid < NG) {
-      sVF[tid] = xvf;
-      sRho[tid] = xrho;
-    }
-  } else
   // ---- one reduction over all groups (fixed order) of V_F w and the density
   // sum; the sums land in sVF and, right after it, sRho (sum |u|^2, without
   // occ) ----
@@ -523,20 +504,64 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         sRho[x] = sum;
     }
   }
-  if (tid < NG) sKH[tid] = khv;
-  lds_barrier();
+  if (!TE && tid < NG) sKH[tid] = khv;
+  lds_barrier();  // V_F w and rho complete; the slot area is free
 
-  // ---- circulant products over all threads: K_H * rho, K_2 * w, K_1 * w ----
-  // thread (p, x) sums the terms y = p, p + P, ... into partials parked in the
-  // slot area (free after the reduction), then thread x adds the P partials
-  // in order.
-  {
-    const int P = blockDim.x / NG;
+  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && obs != nullptr;
+  if constexpr (TE) {
+    // ---- one transform round trip each: group 0 of the first wave on rho,
+    // group 1 on w (the wave's other groups repeat group 1's work and store
+    // nothing); group 1 also sums Ekin and J of the row by Parseval ----
+    if (wave == 0) {
+      const double kA = kxv + Akin, kAo = kxv + Aobs;
+      zd v[R];
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int x = L * m + l;
+        v[m] = gw == 0 ? mk(sRho[x], 0.0) : sW[x];
+      });
+      fft.fwd(v, T, l);
+      double ek = 0.0, jc = 0.0;
+      static_for<0, R>([&](auto jc_) {
+        constexpr int j = decltype(jc_)::value;
+        const double G = gvalue(F::gindex(l, j), NG, a.bx);
+        if (want_obs) {
+          const double p2 = zabs2(v[j]), q = G + kAo;
+          ek = fma(0.5 * q * q, p2, ek);
+          jc = fma(q, p2, jc);
+        }
+        const double q = G + kA;
+        v[j] = zscal(v[j], gw == 0 ? sK2[l * R + j] : (0.5 / NG) * q * q);
+      });
+      fft.inv(v, T, l);
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int x = L * m + l;
+        if (gw == 0) sKH[x] = occ * v[m].x;  // the density sum holds sum |u|^2
+        else if (gw == 1) sK1[x] = v[m];
+      });
+      if (want_obs) {
+#pragma unroll
+        for (int off = 1; off < L; off <<= 1) {
+          ek += __shfl_xor(ek, off);
+          jc += __shfl_xor(jc, off);
+        }
+        if (gw == 1 && l == 0) {
+          sRed[0] = ek / NG;
+          sRed[1] = jc / NG;
+        }
+      }
+    }
+  } else {
+    // ---- circulant products over all threads: K_H * rho, K_2 * w, K_1 * w;
+    // thread (p, x) sums the terms y = p, p + P, ... into partials parked in
+    // the slot area (free after the reduction), then thread x adds the P
+    // partials in order ----
     const int x = tid % NG, p = tid / NG;
     if (p < P) {
       double vh = 0.0;
       zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
-      auto term = [&](int y) {
+      for (int y = p; y < NG; y += P) {
         const int d = (x - y) & (NG - 1);
         const zd wy = sW[y];
         const double k2 = sK2[d];
@@ -548,14 +573,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         c1.x = fma(-k1.y, wy.y, c1.x);
         c1.y = fma(k1.x, wy.y, c1.y);
         c1.y = fma(k1.y, wy.x, c1.y);
-      };
-      if (P == 1) {
-        // one-wave workgroups (batches): NG serial terms, unrolled so that
-        // several terms' LDS reads are in flight
-#pragma unroll 8
-        for (int y = 0; y < NG; ++y) term(y);
-      } else {
-        for (int y = p; y < NG; y += P) term(y);
       }
       double* q = reinterpret_cast<double*>(smem) + 5 * (size_t)tid;
       q[0] = vh;
@@ -568,27 +585,40 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
   lds_barrier();
 
   // ---- epilogue: H w and the mode-specific update ----
-  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && obs != nullptr;
   if (tid < NG) {
     const zd wv = sW[tid];
-    const double* cp = reinterpret_cast<const double*>(smem);
-    double vh = 0.0;
-    zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
-    const int P = blockDim.x / NG;
-    for (int p = 0; p < P; ++p) {
-      const double* q = cp + 5 * (size_t)(p * NG + tid);
-      vh += q[0];
-      c2.x += q[1];
-      c2.y += q[2];
-      c1.x += q[3];
-      c1.y += q[4];
-    }
-    vh *= occ;  // the density sum holds sum |u|^2
-    const double kA = kxv + Akin;
     const zd vf = sVF[tid];
-    // T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w
-    zd h = mk(c2.x + kA * c1.x + 0.5 * kA * kA * wv.x, c2.y + kA * c1.y + 0.5 * kA * kA * wv.y);
-    h = zadd(h, zscal(wv, vh + vionv));
+    double vh, ekt = 0.0, jt = 0.0;  // per-point Ekin and J terms (circulant form)
+    zd tw;
+    if constexpr (TE) {
+      vh = sKH[tid];
+      tw = sK1[tid];
+    } else {
+      const double* cp = reinterpret_cast<const double*>(smem);
+      zd c2 = mk(0.0, 0.0), c1 = mk(0.0, 0.0);
+      vh = 0.0;
+      for (int p = 0; p < P; ++p) {
+        const double* q = cp + 5 * (size_t)(p * NG + tid);
+        vh += q[0];
+        c2.x += q[1];
+        c2.y += q[2];
+        c1.x += q[3];
+        c1.y += q[4];
+      }
+      vh *= occ;  // the density sum holds sum |u|^2
+      const double kA = kxv + Akin;
+      // T w = K_2 * w + (k+A) K_1 * w + (k+A)^2/2 w
+      tw = mk(c2.x + kA * c1.x + 0.5 * kA * kA * wv.x, c2.y + kA * c1.y + 0.5 * kA * kA * wv.y);
+      if (want_obs) {
+        const double kAo = kxv + Aobs;
+        const double w2 = zabs2(wv);
+        const double wc1 = wv.x * c1.x + wv.y * c1.y;  // Re conj(w) (K_1 * w)
+        const double wc2 = wv.x * c2.x + wv.y * c2.y;
+        ekt = wc2 + kAo * wc1 + 0.5 * kAo * kAo * w2;
+        jt = wc1 + kAo * w2;
+      }
+    }
+    zd h = zadd(tw, zscal(wv, vh + vionv));
     h = zadd(h, vf);
     if (a.mode == M_HPSI) {
       gstore(out + o, h);
@@ -609,15 +639,12 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
       }
     }
     if (want_obs) {
-      // per-point terms of Ekin, J, Ex (this row) and N, Eion, EH (row 0)
-      const double kAo = kxv + Aobs;
-      const double w2 = zabs2(wv);
-      const double wc1 = wv.x * c1.x + wv.y * c1.y;  // Re conj(w) (K_1 * w)
-      const double wc2 = wv.x * c2.x + wv.y * c2.y;
+      // per-point terms of Ekin, J (circulant form), Ex (this row) and N,
+      // Eion, EH (row 0)
       const double rho = occ * sRho[tid];
       double* st = reinterpret_cast<double*>(smem) + 5 * (size_t)blockDim.x;
-      st[0 * NG + tid] = wc2 + kAo * wc1 + 0.5 * kAo * kAo * w2;
-      st[1 * NG + tid] = wc1 + kAo * w2;
+      st[0 * NG + tid] = ekt;
+      st[1 * NG + tid] = jt;
       st[2 * NG + tid] = wv.x * vf.x + wv.y * vf.y;
       st[3 * NG + tid] = rho;
       st[4 * NG + tid] = vionv * rho;
@@ -645,6 +672,10 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs a) {
         double s = 0.0;
         for (int c = 0; c < NCH; ++c) s += ch[v * NCH + c];
         t[v] = s;
+      }
+      if constexpr (TE) {  // Ekin and J from the transform of w
+        t[0] = sRed[0];
+        t[1] = sRed[1];
       }
       const double occr = ib < a.nbocc ? occ : 0.0;
       double* p = obs + (size_t)slot * obs_stride;

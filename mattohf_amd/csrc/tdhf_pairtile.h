@@ -14,9 +14,9 @@
 //
 // pairtile_kernel<W> computes, for one trajectory and one tile of TR x TR
 // rows (TR = 8 W; row blocks bi <= bj; W = 8, 4 or 2 waves), every pair once
-// and writes per-row partial sums; the fused apply kernel then adds a row's
-// partials (one per row block, in block order) and continues with the Hartree
-// and kinetic terms and the stage update as for the row-wise exchange.
+// and writes per-row partial sums; pair_epilogue_kernel (below) then adds a
+// row's partials (one per row block, in block order) and applies the Hartree
+// and kinetic terms and the stage update.
 // Smaller tiles give more workgroups for small batches and more partials per
 // row; the batch autotuner chooses.
 //
@@ -258,6 +258,232 @@ __global__ void __launch_bounds__(64 * W) pairtile_kernel(PairTileArgs a) {
       gstore(t.dpart + (size_t)bi * NG + tid, sden);
     }
   }
+}
+
+// Second launch of a pair-tile stage, for every row r = (k,i) of every
+// trajectory: V_F w (the row's partial sums, added in block order) and the
+// density (the trajectory's diagonal-block partials) come from the tiles;
+// the Hartree and kinetic terms are round trips through the group's
+// transform,
+//     vH  = occ IFFT[vCG/NG . FFT[rho]]           (dns_vH, src/Functions.f90:147-162)
+//     T w = IFFT[(G+k+A)^2/(2 NG) . FFT[w]]        (ubk_tubk, src/Functions.f90:212-234);
+// then H w = T w + (vH + vion) w + V_F w (ubkwbk_hwbk, :187-209) and the RK4
+// stage update of RT_RK4_ubk (src/RT_calculation.f90:154-173) or the
+// observables only, as in apply_kernel.  The per-row observables follow by
+// Parseval from FFT[w]:
+//     sum_x conj(w) T_A w      = 1/NG sum_G (G+k+A)^2/2 |w_G|^2   (Ekin),
+//     sum_x Re conj(w) P_A w   = 1/NG sum_G (G+k+A) |w_G|^2       (J).
+// One group of 8 lanes per row, 8 rows per wave, one workgroup barrier (the
+// twiddle table); rows[i] = (trajectory, row) for the nrows rows of the batch.
+__global__ void __launch_bounds__(512) pair_epilogue_kernel(ApplyArgs a, const int2* rows,
+                                                            int nrows) {
+  constexpr int NG = 64;
+  using F = GroupFFT<NG>;
+  constexpr int R = F::R, L = F::L;  // 8 x 8, G layout: register j of lane l holds g = l + 8 j
+  extern __shared__ zd smem[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int l = lane % L, gw = lane / L;
+  const int gi = (blockIdx.x * (int)(blockDim.x >> 6) + wave) * 8 + gw;
+  // a group past the last row repeats the last row and stores nothing, so
+  // that every lane of the wave takes part in the transforms
+  const bool valid = gi < nrows;
+  const int2 e = rows[valid ? gi : nrows - 1];
+  const TrajDesc& t = a.td[e.x];
+  const int row = e.y, nk = t.nk, nbw = a.nbw;
+  const int ik = row / nbw, ib = row - ik * nbw;
+  zd* T = smem + (wave * 8 + gw) * kTilePitch;
+  const zd* w;
+  zd* unext = nullptr;
+  switch (a.mode) {
+    case M_S1: w = t.u0; unext = t.sa; break;
+    case M_S2: w = t.sa; unext = t.sb; break;
+    case M_S3: w = t.sb; unext = t.sa; break;
+    case M_S4: w = t.sa; break;
+    default: w = t.u0; break;
+  }
+  const size_t o = (size_t)NG * row;
+  const double occ = t.occ;
+  // the lane twiddles of the transforms come from an LDS copy of the table
+  // (no registers held across the partial-sum loads)
+  zd* sTw = smem + (blockDim.x >> 3) * kTilePitch;
+  for (int i = tid; i < F::TWN; i += blockDim.x) sTw[i] = gload(a.tw + i);
+
+  // ---- operands: the row, its exchange partials, the density partials ----
+  const F fft{};
+  zd wv[R], vf[R];
+  double rho[R];
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    wv[m] = gload(w + o + L * m + l);
+    vf[m] = mk(0.0, 0.0);
+    rho[m] = 0.0;
+  });
+  // block c's partials are added in block order; block c + 1's are loaded
+  // before that, so two blocks are in flight at a time
+  const int nblk = t.nblk;
+  {
+    zd pa[R], pb[R];
+    double da[R], db[R];
+    auto ld = [&](zd(&p)[R], double(&d)[R], int c) {
+      const zd* xp = t.xpart + ((size_t)row * nblk + c) * NG;
+      const double* dp = t.dpart + (size_t)c * NG;
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        p[m] = gload(xp + L * m + l);
+        d[m] = gload(dp + L * m + l);
+      });
+    };
+    auto add = [&](const zd(&p)[R], const double(&d)[R]) {
+      static_for<0, R>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        vf[m] = zadd(vf[m], p[m]);
+        rho[m] += d[m];
+      });
+    };
+    if (nblk > 0) ld(pa, da, 0);
+    for (int c = 0; c < nblk; c += 2) {
+      if (c + 1 < nblk) ld(pb, db, c + 1);
+      add(pa, da);
+      if (c + 1 >= nblk) break;
+      if (c + 2 < nblk) ld(pa, da, c + 2);
+      add(pb, db);
+    }
+  }
+  const double kxv = gload(t.kx + ik);
+  lds_barrier();  // twiddle table in LDS
+
+  // field values of this launch (as in apply_kernel)
+  int slot = a.obs_slot;
+  double Akin = a.A, Aobs = a.Aobs;
+  if (a.step_base) {
+    const int step = *a.step_base + a.jstep;
+    const double At = gload(t.A + step), Aav = gload(t.Aave + step);
+    if (a.mode == M_S1) {
+      Akin = 2.0 * Aav - At;  // A(it-1) as in RT_RK4_ubk
+      Aobs = step > 0 ? gload(t.A + step - 1) : 0.0;
+      slot = step;
+    } else if (a.mode == M_S2 || a.mode == M_S3) {
+      Akin = Aav;
+    } else if (a.mode == M_S4) {
+      Akin = At;
+    } else {  // M_OBS after step 'step'
+      Akin = At;
+      Aobs = At;
+      slot = step + 1;
+    }
+  }
+  const bool want_obs = (a.mode == M_S1 || a.mode == M_OBS) && t.obs != nullptr;
+
+  // ---- Hartree potential of the trajectory's density ----
+  double vh[R];
+  {
+    zd hv[R];
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      hv[m] = mk(rho[m], 0.0);
+    });
+    fft.fwd_lds(hv, T, l, sTw);
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      hv[j] = zscal(hv[j], gload(t.vhg + l * R + j));
+    });
+    fft.inv_lds(hv, T, l, sTw);
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      vh[m] = occ * hv[m].x;
+    });
+  }
+
+  // ---- kinetic term, and Ekin and J of the row by Parseval ----
+  zd tw[R];
+  double ek = 0.0, jc = 0.0;
+  {
+    const double kA = kxv + Akin, kAo = kxv + Aobs;
+    static_for<0, R>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      tw[m] = wv[m];
+    });
+    fft.fwd_lds(tw, T, l, sTw);
+    static_for<0, R>([&](auto jc_) {
+      constexpr int j = decltype(jc_)::value;
+      const double G = gvalue(F::gindex(l, j), NG, a.bx);
+      if (want_obs) {
+        const double p2 = zabs2(tw[j]), q = G + kAo;
+        ek = fma(0.5 * q * q, p2, ek);
+        jc = fma(q, p2, jc);
+      }
+      const double q = G + kA;
+      tw[j] = zscal(tw[j], (0.5 / NG) * q * q);
+    });
+    fft.inv_lds(tw, T, l, sTw);
+  }
+
+  // ---- H w and the stage update ----
+  const int mode = a.mode;
+  double ex = 0.0, sn = 0.0, si = 0.0, sh = 0.0;
+  static_for<0, R>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const int x = L * m + l;
+    const double vio = gload(t.vion + x);
+    const zd h = zadd(zadd(tw[m], zscal(wv[m], vh[m] + vio)), vf[m]);
+    if (want_obs) {
+      const double r = occ * rho[m];
+      ex += wv[m].x * vf[m].x + wv[m].y * vf[m].y;
+      sn += r;
+      si += vio * r;
+      sh += vh[m] * r;
+    }
+    if (valid && mode >= M_S1 && mode <= M_S4) {
+      const zd kk = mk(h.y, -h.x);  // -i * H w
+      const double hdt = 0.5 * a.dt;
+      const zd u0v = mode == M_S1 ? wv[m] : gload(t.u0 + o + x);
+      if (mode == M_S1) {
+        gstore(t.ksum + o + x, kk);
+        gstore(unext + o + x, mk(u0v.x + hdt * kk.x, u0v.y + hdt * kk.y));
+      } else {
+        const zd ksv = gload(t.ksum + o + x);
+        if (mode == M_S2 || mode == M_S3) {
+          gstore(t.ksum + o + x, mk(ksv.x + 2.0 * kk.x, ksv.y + 2.0 * kk.y));
+          const double c = mode == M_S2 ? hdt : a.dt;
+          gstore(unext + o + x, mk(u0v.x + c * kk.x, u0v.y + c * kk.y));
+        } else {
+          gstore(t.u0 + o + x, mk(u0v.x + (ksv.x + kk.x) * a.dt / 6.0,
+                                  u0v.y + (ksv.y + kk.y) * a.dt / 6.0));
+        }
+      }
+    }
+  });
+  if (want_obs) {
+    // sums over the group's 8 lanes (the same value in every lane)
+#pragma unroll
+    for (int off = 1; off < L; off <<= 1) {
+      ek += __shfl_xor(ek, off);
+      jc += __shfl_xor(jc, off);
+      ex += __shfl_xor(ex, off);
+      sn += __shfl_xor(sn, off);
+      si += __shfl_xor(si, off);
+      sh += __shfl_xor(sh, off);
+    }
+    if (valid && l == 0) {
+      const double occr = ib < a.nbocc ? occ : 0.0;
+      double* p = t.obs + (size_t)slot * t.obs_stride;
+      gstore(p + 3 * row + 0, occr * (ek / NG) * a.dv);
+      gstore(p + 3 * row + 1, occr * (jc / NG));
+      gstore(p + 3 * row + 2, occr * ex * a.dv);
+      if (row == 0) {
+        const int rows_t = nbw * nk;
+        gstore(p + 3 * rows_t + 0, sn * a.dv);
+        gstore(p + 3 * rows_t + 1, si * a.dv);
+        gstore(p + 3 * rows_t + 2, 0.5 * sh * a.dv);
+      }
+    }
+  }
+}
+
+// LDS of pair_epilogue_kernel with nw waves: one transpose tile per group
+// and the twiddle table of GroupFFT<64>.
+__host__ __device__ constexpr size_t pair_epilogue_lds_bytes(int nw) {
+  return ((size_t)nw * 8 * kTilePitch + GroupFFT<64>::TWN) * sizeof(zd);
 }
 
 }  // namespace tdhf
