@@ -25,6 +25,7 @@
 #include "tdhf_pairtile.h"
 #include "tdhf_gs.h"
 #include "tdhf_adjoint.h"
+#include "tdhf_adjtile.h"
 #include "tdhf_post.h"
 
 using tdhf::zd;

@@ -58,6 +58,14 @@ struct AdjArgs {
   size_t tstride;
   int fstride;
   size_t abar_tstride;
+  // exchange part of J^T gbar, density and s from the pair tiles
+  // (adjtile_kernel; adjoint_kernel<NG, true>): nblk partial sums per row,
+  // trajectory t at t * xstride, and per trajectory nblk density partials
+  // followed by nblk partials of s, at t * dstride
+  const zd* xpart;
+  const double* dpart;
+  int nblk;
+  size_t xstride, dstride;
 };
 
 // Per-group LDS slot of adjoint_kernel (complex elements): the transpose
@@ -92,7 +100,9 @@ __host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
          (adjoint_lds_twiddles<NG>() ? GroupFFT<NG>::TWN * sizeof(zd) : 0);
 }
 
-template <int NG>
+// XP: the exchange part, the density and s come from the pair tiles' partial
+// sums (adjtile_kernel): the pair loop is compiled out.
+template <int NG, bool XP = false>
 __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   // per-trajectory view of the arguments
   AdjArgs a = a0;
@@ -166,8 +176,50 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   F fft;
   if constexpr (LTW) {
     for (int i = tid; i < F::TWN; i += blockDim.x) sTw[i] = gload(a.tw + i);
-  } else {
+  } else if constexpr (!XP) {
     fft.init(a.tw, l);
+  }
+  // pair tiles: the exchange part, the density and s of this row from their
+  // partials.  The H = blockDim / NG threads of a grid point each add a
+  // contiguous run of the blocks in order, eight blocks in flight, into the
+  // slot area (free: no pair loop); the runs are added in order after the
+  // barrier below.
+  if constexpr (XP) {
+    const int H = blockDim.x / NG, x = tid % NG, h = tid / NG;
+    if (h < H) {
+      const int nblk = a.nblk, c1 = nblk * (h + 1) / H;
+      const zd* xp = a.xpart + (size_t)tr * a.xstride + (size_t)row * nblk * NG + x;
+      const double* dp = a.dpart + (size_t)tr * a.dstride + x;
+      zd xacc = mk(0.0, 0.0);
+      double xrho = 0.0, xs = 0.0;
+      int c = nblk * h / H;
+      for (; c + 8 <= c1; c += 8) {
+        zd q[8];
+        double d[8], e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          q[i] = gload(xp + (size_t)(c + i) * NG);
+          d[i] = gload(dp + (size_t)(c + i) * NG);
+          e[i] = gload(dp + (size_t)(nblk + c + i) * NG);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          xacc = zadd(xacc, q[i]);
+          xrho += d[i];
+          xs += e[i];
+        }
+      }
+      for (; c < c1; ++c) {
+        xacc = zadd(xacc, gload(xp + (size_t)c * NG));
+        xrho += gload(dp + (size_t)c * NG);
+        xs += gload(dp + (size_t)(nblk + c) * NG);
+      }
+      double* hp = reinterpret_cast<double*>(smem) + (size_t)4 * NG * h;
+      hp[x] = xacc.x;
+      hp[NG + x] = xacc.y;
+      hp[2 * NG + x] = xrho;
+      hp[3 * NG + x] = xs;
+    }
   }
   auto fwd = [&](zd (&v)[R]) {
     if constexpr (LTW) fft.fwd_lds(v, T, l, sTw); else fft.fwd(v, T, l);
@@ -189,7 +241,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   lds_barrier();
 
   // ---- pair loop over all partner rows (q, j) ----
-  const int nitem = a.nk * a.nb;
+  const int nitem = XP ? 0 : a.nk * a.nb;  // pair tiles: no pairs here
   for (int item = grp; item < nitem; item += ngrp) {
     const int iq = item / a.nb, jb = item - iq * a.nb;
     const size_t prow = (size_t)NG * (jb + a.nb * iq);
@@ -274,7 +326,22 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
 
   // ---- one reduction over the groups (fixed order): exchange sum, density
   // and s land in sACC and, right after it, sRho and sS ----
-  {
+  if constexpr (XP) {
+    if (tid < NG) {
+      const int H = blockDim.x / NG;
+      const double* hp = reinterpret_cast<const double*>(smem);
+      zd xa = mk(0.0, 0.0);
+      double xr = 0.0, xv = 0.0;
+      for (int h = 0; h < H; ++h, hp += 4 * NG) {
+        xa = zadd(xa, mk(hp[tid], hp[NG + tid]));
+        xr += hp[2 * NG + tid];
+        xv += hp[3 * NG + tid];
+      }
+      sACC[tid] = xa;
+      sRho[tid] = xr;
+      sS[tid] = xv;
+    }
+  } else {
     static_assert((4 * R) % GPW == 0, "four sums per point split over the groups");
     // the groups of a wave are summed in registers, then the nw wave partials
     // through LDS, each parked in its own wave's slots as [re | im | rho | s][x]

@@ -40,6 +40,13 @@ struct AdjState {
   double* res = nullptr;   // ntraj * (nt+1) * 7 reduced observables
   double* abar = nullptr;  // ntraj * seg * 4 * rows per-row field derivatives
   double* stsum = nullptr; // ntraj * nt * 4 stage field derivatives
+  // reverse stages by pair tiles (NG = 128, NB == NBocc; adjtile_kernel):
+  // per-row partials, density and s partials, the tile list, a zero row
+  zd* xpart = nullptr;
+  double* dpart = nullptr;
+  int4* tiles = nullptr;
+  double* zrow = nullptr;
+  int ntiles = 0, nblk = 0;
   // captured step blocks, keyed by the buffers they use
   std::vector<std::pair<std::vector<const void*>, hipGraphExec_t>> graphs;
 };
@@ -80,6 +87,10 @@ void adj_free(AdjState& s) {
   dfree(s.res);
   dfree(s.abar);
   dfree(s.stsum);
+  dfree(s.xpart);
+  dfree(s.dpart);
+  dfree(s.tiles);
+  dfree(s.zrow);
   for (auto& g : s.graphs) (void)hipGraphExecDestroy(g.second);
   s = AdjState();
 }
@@ -124,8 +135,68 @@ size_t adj_lds_for(int ng, int nw) {
   return 0;
 }
 
-int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj) {
+// Waves per workgroup of the reverse stage's second launch after the pair
+// tiles (no pair loop): two threads per grid point for the circulant products.
+constexpr int kAdjTileEpiWaves = 4;
+constexpr int kAdjTileWaves = 8;  // adjtile_kernel<8>: 32-row tiles
+
+// One reverse stage.  With ts holding pair tiles (adj_tiles_setup) and a
+// stage mode, the exchange part comes from adjtile_kernel and
+// adjoint_kernel<128, true> finishes the stage; otherwise adjoint_kernel
+// evaluates every ordered pair itself.
+int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj, const AdjState* ts = nullptr) {
   const int rows = c->nb * c->nk;
+  if (ts && ts->tiles && a.mode >= tdhf::A_S1 && a.mode <= tdhf::A_S4) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+      if (c->ev_used + 2 > c->ev_pool.size())
+        for (int i = 0; i < 64; ++i) {
+          hipEvent_t e;
+          HIPCHK(c, hipEventCreate(&e));
+          c->ev_pool.push_back(e);
+        }
+      e0 = c->ev_pool[c->ev_used++];
+      e1 = c->ev_pool[c->ev_used++];
+      HIPCHK(c, hipEventRecord(e0, c->stream));
+    }
+    tdhf::AdjTileArgs t{};
+    t.nk = c->nk;
+    t.nb = c->nb;
+    t.vtl = c->d_vtl;
+    t.tw = c->d_tw;
+    t.zrow = ts->zrow;
+    t.tiles = ts->tiles;
+    t.y = a.y;
+    t.gb = a.gb;
+    t.step_base = a.step_base;
+    t.jstep = a.jstep;
+    t.tape = a.tape;
+    t.tape_seg = a.tape_seg;
+    t.blk = a.blk;
+    t.mode = a.mode;
+    t.tstride = a.tstride;
+    t.xpart = ts->xpart;
+    t.dpart = ts->dpart;
+    t.xstride = (size_t)rows * ts->nblk * c->ng;
+    t.dstride = (size_t)2 * ts->nblk * c->ng;
+    t.nblk = ts->nblk;
+    hipLaunchKernelGGL(tdhf::adjtile_kernel<kAdjTileWaves>, dim3(ts->ntiles),
+                       dim3(64 * kAdjTileWaves), tdhf::adjtile_lds_bytes(kAdjTileWaves), c->stream,
+                       t);
+    HIPCHK(c, hipGetLastError());
+    tdhf::AdjArgs b = a;
+    b.xpart = t.xpart;
+    b.dpart = t.dpart;
+    b.nblk = t.nblk;
+    b.xstride = t.xstride;
+    b.dstride = t.dstride;
+    hipLaunchKernelGGL((tdhf::adjoint_kernel<128, true>), dim3(rows, ntraj),
+                       dim3(64 * kAdjTileEpiWaves), adj_lds_for(128, kAdjTileEpiWaves), c->stream,
+                       b);
+    HIPCHK(c, hipGetLastError());
+    if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
+    return 0;
+  }
   // the context's launch shape also for large uniform batches: at NG=128
   // with 8 pulses, 2 or 4 waves per workgroup measured 15 % and 37 % slower
   // than 8 (unlike the forward kernel, see launch_apply)
@@ -173,6 +244,56 @@ int adj_prepare(tdhf_ctx* c) {
   }
   if (lds > 64 * 1024)
     HIPCHK(c, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return 0;
+}
+
+// Pair tiles for the reverse stages (NG = 128, every band occupied): the
+// partial buffers and the tile list of the record's trajectories, set up once
+// before the first reverse sweep.  Without device memory for them the
+// reverse stages stay row-wise (the same results to rounding).
+int adj_tiles_setup(tdhf_ctx* c, AdjState& s) {
+  if (c->ng != 128 || c->nb != c->nbocc || s.tiles) return 0;
+  const int rows = c->nb * c->nk;
+  if ((long)rows >= (1L << 22)) return 0;  // the kernel's float k-point division
+  constexpr int TR = tdhf::adjtile_rows(kAdjTileWaves);
+  const int nblk = (rows + TR - 1) / TR;
+  std::vector<int4> map;
+  for (int pass = 0; pass < 2; ++pass)  // off-diagonal tiles (the costlier) first
+    for (int t = 0; t < s.ntraj; ++t)
+      for (int bi = 0; bi < nblk; ++bi)
+        for (int bj = bi; bj < nblk; ++bj)
+          if ((bi == bj) == (pass == 1)) map.push_back(make_int4(t, bi, bj, 0));
+  const size_t xs = (size_t)s.ntraj * rows * nblk * c->ng;
+  const size_t ds = (size_t)s.ntraj * 2 * nblk * c->ng;
+  auto get = [](void** p, size_t bytes) { return hipMalloc(p, bytes + 16) == hipSuccess; };
+  zd* xp = nullptr;
+  double *dp = nullptr, *zr = nullptr;
+  int4* tl = nullptr;
+  if (!get((void**)&xp, xs * sizeof(zd)) || !get((void**)&dp, ds * sizeof(double)) ||
+      !get((void**)&tl, map.size() * sizeof(int4)) || !get((void**)&zr, c->ng * sizeof(double))) {
+    dfree(xp);
+    dfree(dp);
+    dfree(tl);
+    dfree(zr);
+    (void)hipGetLastError();  // the failed allocation is not an error of the call
+    return 0;
+  }
+  s.xpart = xp;
+  s.dpart = dp;
+  s.tiles = tl;
+  s.zrow = zr;
+  s.ntiles = (int)map.size();
+  s.nblk = nblk;
+  HIPCHK(c, hipMemcpyAsync(tl, map.data(), map.size() * sizeof(int4), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemsetAsync(zr, 0, c->ng * sizeof(double), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipFuncSetAttribute((const void*)tdhf::adjtile_kernel<kAdjTileWaves>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)tdhf::adjtile_lds_bytes(kAdjTileWaves)));
+  HIPCHK(c, hipFuncSetAttribute((const void*)tdhf::adjoint_kernel<128, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)adj_lds_for(128, kAdjTileEpiWaves)));
   return 0;
 }
 
@@ -312,7 +433,7 @@ int adj_backward_steps(tdhf_ctx* c, AdjState& s, int s0, int n, zd** ub, zd** ua
         a.mode = st;
         a.gb = *pga;
         a.gout = *pgb;
-        CHKRET(launch_adj(c, a, s.ntraj));
+        CHKRET(launch_adj(c, a, s.ntraj, &s));
         std::swap(*pga, *pgb);
       }
       // current-squared objective: J after step it-1 is evaluated on y1 of step it
@@ -511,6 +632,7 @@ extern "C" int tdhf_bwd_batch(tdhf_ctx* c, double* dLdA, double* dLdu0) {
   if (!s.buf) CHKRET(dalloc(c, &s.buf, 4 * blk));
   if (!s.abar) CHKRET(dalloc(c, &s.abar, (size_t)s.ntraj * s.seg * 4 * rows));
   if (!s.stsum) CHKRET(dalloc(c, &s.stsum, (size_t)s.ntraj * nt * 4));
+  CHKRET(adj_tiles_setup(c, s));
   zd *ua = s.buf, *ub = s.buf + blk, *ga = s.buf + 2 * blk, *gb = s.buf + 3 * blk;
   // cotangent of the final state (s.st holds it after the forward run)
   if (s.objective == TDHF_OBJ_ENERGY) {

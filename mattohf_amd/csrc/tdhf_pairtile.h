@@ -58,13 +58,14 @@ __host__ __device__ constexpr size_t pairtile_lds_bytes(int W) {
   return (size_t)2 * tile_rows(W) * kTilePitch * sizeof(zd);
 }
 
-// The passes of a diagonal tile of W waves, as bit masks over p = 8 p1 + p0
-// (p1 < W, p0 < 8): the offsets o = (p1, p0) with o <= -o (mod W, mod 8)
-// run, two-sided unless o == -o.
-__host__ __device__ constexpr uint64_t diag_pass_mask(int W, bool two_sided) {
+// The passes of a diagonal tile of W waves of G groups each (G = 8 at NG = 64,
+// 4 at NG = 128), as bit masks over p = G p1 + p0 (p1 < W, p0 < G): the
+// offsets o = (p1, p0) with o <= -o (mod W, mod G) run, two-sided unless
+// o == -o.
+__host__ __device__ constexpr uint64_t diag_pass_mask(int W, bool two_sided, int G = 8) {
   uint64_t m = 0;
-  for (int p = 0; p < 8 * W; ++p) {
-    const int p1 = p >> 3, p0 = p & 7, m1 = (W - p1) & (W - 1), m0 = (8 - p0) & 7;
+  for (int p = 0; p < G * W; ++p) {
+    const int p1 = p / G, p0 = p % G, m1 = (W - p1) & (W - 1), m0 = (G - p0) & (G - 1);
     const bool self = p1 == m1 && p0 == m0;
     const bool run = self || p1 < m1 || (p1 == m1 && p0 < m0);
     if (run && (!two_sided || !self)) m |= uint64_t(1) << p;
@@ -78,6 +79,10 @@ static_assert(__builtin_popcountll(diag_pass_mask(8, false)) == diag_passes(8) &
                   __builtin_popcountll(diag_pass_mask(2, false)) == diag_passes(2) &&
                   __builtin_popcountll(diag_pass_mask(8, true)) == diag_passes(8) - 4,
               "diagonal tiles: 4 W + 2 passes, four of them one-sided");
+static_assert(__builtin_popcountll(diag_pass_mask(8, false, 4)) == 2 * 8 + 2 &&
+                  __builtin_popcountll(diag_pass_mask(4, false, 4)) == 2 * 4 + 2 &&
+                  __builtin_popcountll(diag_pass_mask(8, true, 4)) == 2 * 8 - 2,
+              "diagonal tiles of 4 groups per wave: 2 W + 2 passes, four one-sided");
 
 template <int W>
 __global__ void __launch_bounds__(64 * W) pairtile_kernel(PairTileArgs a) {

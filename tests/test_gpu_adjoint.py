@@ -47,6 +47,25 @@ def test_adjoint_matches_oracle(case):
     assert np.abs(gu - guo).max() < 1e-9 * max(1.0, np.abs(guo).max())
 
 
+def test_adjoint_pair_tiles_match_oracle():
+    """NG = 128 with every band occupied runs the reverse stages through the
+    pair tiles (32-row tiles): 48 rows make one off-diagonal tile and two
+    diagonal ones, the second block partly empty."""
+    ng, nk, nb, a, z, nt, dt = 128, 8, 6, 12.0, 12.0, 4, 0.004
+    S = po.System(ng, nk, nb, nb, a)
+    vion = S.ion_potential([z], [1.0], [0.0])
+    O = po.Oracle(S, vion)
+    u0 = O.GS_calculation(S.gram_schmidt(S.init_wf()), 8, 2)[0]
+    A = 0.1 * np.random.default_rng(3).standard_normal(nt)
+    Lo, go, guo = O.adjoint_energy(u0, dt, A)
+    with T.TDHF(ng, nk, nb, nb, a, dt, vion) as G:
+        L, uf, e, n, j = G.fwd(u0, A)
+        g, gu = G.bwd()
+    assert abs(L - Lo) < 1e-11
+    assert np.abs(g - go).max() < 1e-9 * max(1.0, np.abs(go).max())
+    assert np.abs(gu - guo).max() < 1e-9 * max(1.0, np.abs(guo).max())
+
+
 @pytest.mark.parametrize("case", CASES[:2], ids=lambda c: f"ng{c[0]}_nk{c[1]}")
 def test_adjoint_finite_differences(case):
     ng, nk, nb, a, z, nt, dt = case
