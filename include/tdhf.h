@@ -243,6 +243,11 @@ int tdhf_bwd(tdhf_ctx* ctx, double* dLdA, double* dLdu0);
  * initial orbitals and field each; same NK, tables and ion potential), one
  * launch per RK4 stage for all of them (adjoint stages likewise), with either
  * objective.  The memory budget covers the stage states of all trajectories.
+ * At NG = 128 the backward call evaluates the reverse stages by pair tiles
+ * (one pair of transforms per unordered pair of rows) and allocates their
+ * partial sums outside the budget, ntraj * NK*NB * ceil(NK*NB/32) * NG * 16
+ * bytes (2.1 GB for 32 trajectories at NK = 128, NB = 8); without that
+ * memory it evaluates them row-wise (the same results to rounding).
  *   u      : ntraj states concatenated (NG*NB*NK complex each), in: initial,
  *            out: final
  *   A      : ntraj*nt, trajectory t at A + nt*t
