@@ -180,10 +180,10 @@ __global__ void __launch_bounds__(64 * W) adjtile_kernel(AdjTileArgs a) {
       v[j] = zscal(v[j], vtr[j]);
     });
     fft.inv(v, T, l);
-    // y_s is read again here (a cache hit) rather than held through the
-    // transforms; h = y_r conj y_s replaces At in place
-    // row s's sum; a one-sided pass (diagonal tiles) writes a spare row that
-    // is never read instead, so the update needs no branch
+    // y_s is read again below (a cache hit) rather than held through the
+    // transforms, and h = y_r conj y_s replaces At in place.  Row s's sum: a
+    // one-sided pass (diagonal tiles) writes a spare row that is never read
+    // instead, so the update needs no branch.
     zd* js = acj + (size_t)(both ? s - TR * bj : TR) * kAdjTilePitch;
     {
       const double fx = sgn * dnyq * px, fy = sgn * dnyq * py;
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(64 * W) adjtile_kernel(AdjTileArgs a) {
         const zd ys = gload(ysp + L * m + l);
         const zd at = mk(v[m].x + fx, v[m].y + fy);
         acc[m] = zadd(acc[m], zmul(at, ys));  // row r: y_s At
-        {                           // row s: y_r conj(At)
+        {  // row s: y_r conj(At)
           zd sm = js[L * m + l];
           sm = zadd(sm, zmulc(yr[m], at));
           js[L * m + l] = sm;
@@ -200,13 +200,18 @@ __global__ void __launch_bounds__(64 * W) adjtile_kernel(AdjTileArgs a) {
         v[m] = zmulc(yr[m], ys);
       });
     }
-    // ---- Bt: h = y_r conj y_s
+    // ---- Bt: h = y_r conj y_s (the table row read again before the
+    // transform, so that the read is covered by it)
+    static_for<0, R>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      vtr[j] = gload(vt + j);
+    });
     fft.fwd(v, T, l);
     const double nx = __shfl(v[JNYQ].x, gw * L, 64);
     const double ny = __shfl(v[JNYQ].y, gw * L, 64);
     static_for<0, R>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      v[j] = zscal(v[j], gload(vt + j));
+      v[j] = zscal(v[j], vtr[j]);
     });
     fft.inv(v, T, l);
     {
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(64 * W) adjtile_kernel(AdjTileArgs a) {
         constexpr int m = decltype(mc)::value;
         const zd gs = gload(gsp + L * m + l);
         acc[m] = zadd(acc[m], zmul(mk(v[m].x + fx, v[m].y + fy), gs));  // row r: gbar_s Bt
-        {                                                     // row s: gbar_r conj(C h)
+        {  // row s: gbar_r conj(C h)
           zd sm = js[L * m + l];
           sm = zadd(sm, zmulc(gr[m], v[m]));
           js[L * m + l] = sm;
