@@ -42,7 +42,7 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes per apply_kernel<64> launch of the plain workload, from rocprofv3
 # FETCH_SIZE / WRITE_SIZE per dispatch scaled by a plain 1 GiB device copy
 # (FETCH x2.0, WRITE x1.0): profiles/apply_kernel_ng64_nk64_nb4_counters.md
-TRAFFIC_KIB = {"fetch_kib": 1577.5, "fetch_scale": 2.0, "write_kib": 449.8, "write_scale": 1.0}
+TRAFFIC_KIB = {"fetch_kib": 1568.4, "fetch_scale": 2.0, "write_kib": 449.8, "write_scale": 1.0}
 
 CFG = dict(acell=10.0, ng=64, nk=64, nb=4, nbocc=4, z=8.0, alpha=1.0, dt=0.01, A0=0.3,
            omega=0.057, T=400.0)
