@@ -158,12 +158,19 @@ def test_jsq_finite_differences():
         assert abs(fd - np.real(np.vdot(gu[0], du))) <= 1e-6 * abs(fd) + 1e-13
 
 
+# the second case runs its reverse stages through the pair tiles (NG = 128,
+# 48 rows: three tiles per trajectory), so the per-trajectory offsets of the
+# tile kernel and its partial sums are checked against single runs
+BATCH_CASES = [CASES[1], (128, 8, 6, 12.0, 12.0, 4, 0.004)]
+
+
+@pytest.mark.parametrize("case", BATCH_CASES, ids=lambda c: f"ng{c[0]}_nk{c[1]}")
 @pytest.mark.parametrize("objective", ["energy", "jsq"])
-def test_batch_equals_single_runs(objective):
+def test_batch_equals_single_runs(objective, case):
     """Three trajectories (own pulse, own initial state) in one launch per stage
     give each trajectory's single-run objective and gradient."""
-    ng, nk, nb, a, z, nt, dt = CASES[1]
-    S, O, vion, u0, A0 = setup_case(*CASES[1])
+    ng, nk, nb, a, z, nt, dt = case
+    S, O, vion, u0, A0 = setup_case(*case)
     rng = np.random.default_rng(5)
     As = np.stack([A0, 0.1 * rng.standard_normal(nt), 0.05 * rng.standard_normal(nt)])
     us = [u0, S.gram_schmidt(u0 + 0.05 * rng.standard_normal(S.shape)), u0]
