@@ -79,6 +79,17 @@ __host__ __device__ constexpr int adjoint_slot() {
   return need + ((8 - need % 16) + 16) % 16;
 }
 
+// Slot of the launches that take the exchange part from the pair tiles
+// (adjoint_kernel<NG, true>: no transposes, no reduction partials): just the
+// circulant-product partials and the staging at the smallest launch, so that
+// more workgroups share a CU.
+template <int NG>
+__host__ __device__ constexpr int adjoint_epi_slot() {
+  constexpr int nwmin = NG > 64 ? NG / 64 : 1, gpw = 64 / GroupFFT<NG>::L;
+  constexpr int need = (8 * 64 * nwmin + NG + 2 * gpw * nwmin - 1) / (2 * gpw * nwmin);
+  return need + ((8 - need % 16) + 16) % 16;
+}
+
 // LDS of adjoint_kernel<NG> with nw waves (bytes): the groups' tiles (pair
 // transposes, then reduction partials, circulant-product partials and the
 // field derivative staging), y, gbar, the exchange sum and K_1 (complex NG
@@ -93,8 +104,9 @@ __host__ __device__ constexpr bool adjoint_lds_twiddles() {
 }
 
 template <int NG>
-__host__ __device__ constexpr size_t adjoint_lds_bytes(int nw) {
-  return (size_t)nw * (64 / GroupFFT<NG>::L) * adjoint_slot<NG>() * sizeof(zd) +
+__host__ __device__ constexpr size_t adjoint_lds_bytes(int nw, bool epilogue_only = false) {
+  return (size_t)nw * (64 / GroupFFT<NG>::L) *
+             (epilogue_only ? adjoint_epi_slot<NG>() : adjoint_slot<NG>()) * sizeof(zd) +
          4 * NG * sizeof(zd) + 4 * NG * sizeof(double) + 16 * sizeof(double) +
          ((size_t)64 * nw > 4 * NG ? (size_t)64 * nw : 4 * NG) * sizeof(double) +
          (adjoint_lds_twiddles<NG>() ? GroupFFT<NG>::TWN * sizeof(zd) : 0);
@@ -142,7 +154,7 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   const bool egrad = a.mode == A_EGRAD;
   const zd* gbar = egrad ? a.y : a.gb;
 
-  constexpr int SSZ = adjoint_slot<NG>();
+  constexpr int SSZ = XP ? adjoint_epi_slot<NG>() : adjoint_slot<NG>();
   zd* T = smem + grp * SSZ;
   zd* sY = smem + ngrp * SSZ;      // y_ki
   zd* sG = sY + NG;                // gbar_ki

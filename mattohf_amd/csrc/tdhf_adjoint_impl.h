@@ -191,8 +191,8 @@ int launch_adj(tdhf_ctx* c, const tdhf::AdjArgs& a, int ntraj, const AdjState* t
     b.xstride = t.xstride;
     b.dstride = t.dstride;
     hipLaunchKernelGGL((tdhf::adjoint_kernel<128, true>), dim3(rows, ntraj),
-                       dim3(64 * kAdjTileEpiWaves), adj_lds_for(128, kAdjTileEpiWaves), c->stream,
-                       b);
+                       dim3(64 * kAdjTileEpiWaves),
+                       tdhf::adjoint_lds_bytes<128>(kAdjTileEpiWaves, true), c->stream, b);
     HIPCHK(c, hipGetLastError());
     if (c->timing) HIPCHK(c, hipEventRecord(e1, c->stream));
     return 0;
@@ -293,7 +293,7 @@ int adj_tiles_setup(tdhf_ctx* c, AdjState& s) {
                                 (int)tdhf::adjtile_lds_bytes(kAdjTileWaves)));
   HIPCHK(c, hipFuncSetAttribute((const void*)tdhf::adjoint_kernel<128, true>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)adj_lds_for(128, kAdjTileEpiWaves)));
+                                (int)tdhf::adjoint_lds_bytes<128>(kAdjTileEpiWaves, true)));
   return 0;
 }
 
