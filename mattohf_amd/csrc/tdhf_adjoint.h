@@ -94,7 +94,7 @@ __host__ __device__ constexpr int adjoint_epi_slot() {
 // transposes, then reduction partials, circulant-product partials and the
 // field derivative staging), y, gbar, the exchange sum and K_1 (complex NG
 // each), rho, s, K_H, K_2 (NG doubles each), a small scratch and the chunk-sum
-// scratch.
+// scratch (epilogue_only: the smaller slots of adjoint_kernel<NG, true>).
 // NG = 32 reads the transforms' lane twiddles from an LDS copy of the table:
 // its geometry (two register sets of twiddles) would otherwise push the
 // kernel's registers into spills.
@@ -170,6 +170,9 @@ __global__ void __launch_bounds__(512) adjoint_kernel(AdjArgs a0) {
   zd* sTw = reinterpret_cast<zd*>(sPart + (64 * nw > 4 * NG ? 64 * nw : 4 * NG));
   static_assert(8 * 64 * (NG > 64 ? NG / 64 : 1) + NG <= 2 * SSZ * GPW * (NG > 64 ? NG / 64 : 1),
                 "slot area too small for the epilogue scratch");
+  // XP: the runs of partial sums, 4 NG doubles per thread of a grid point
+  // (256 per wave), also fit the slot area
+  static_assert(!XP || 128 <= SSZ * GPW, "slot area too small for the partial-sum runs");
 
   const size_t o = (size_t)NG * row + (tid < NG ? tid : 0);
   zd yv = mk(0.0, 0.0), gv = mk(0.0, 0.0);
